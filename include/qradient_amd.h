@@ -1,0 +1,149 @@
+/*
+ * qradient_amd -- C ABI of the MI355X (gfx950) statevector grad_run library.
+ *
+ * The reference (frederikwilde/qradient) is pure Python and has no FFI; its
+ * boundary is the Python object protocol of the circuit classes.  Each entry
+ * point below replaces one method of that protocol (reference file:line in
+ * the comment); the Python package qradient_amd binds them with ctypes
+ * (INTEGRATION.md shows the binding).
+ *
+ * Conventions
+ *  - every function returns QG_OK (0) or a QG_ERR_* code; qg_last_error()
+ *    returns a thread-local message for the last failure on this thread;
+ *  - pointers are host pointers owned by the caller unless the name says
+ *    "dev"; arrays are C-contiguous;
+ *  - complex amplitudes cross the ABI as interleaved (re, im) float64 pairs;
+ *  - qubit q is index bit (n-1-q): qubit 0 is the most significant bit
+ *    (reference qradient/physical_components/state.py:84-87);
+ *  - a qg_ctx and the plans/states made from it are single-threaded objects:
+ *    calls on one context must not overlap; separate contexts may be used
+ *    from separate threads.
+ */
+#ifndef QRADIENT_AMD_H
+#define QRADIENT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QG_OK 0
+#define QG_ERR_INVALID 1     /* bad argument: maps to ValueError in Python */
+#define QG_ERR_UNSUPPORTED 2 /* valid request this build does not support */
+#define QG_ERR_HIP 3         /* HIP runtime failure */
+#define QG_ERR_NOMEM 4       /* device memory exhausted */
+
+#define QG_C128 0 /* complex128 state, FP64 arithmetic ("plain" method) */
+#define QG_C64 1  /* complex64 state, FP64 arithmetic and FP64 reductions */
+
+#define QG_ENT_CNOT 0 /* CNOT ladder of state.py:209-251 (odd bonds act first) */
+#define QG_ENT_CZ 1   /* nearest-neighbour CZ ladder (diagonal variant) */
+
+#define QG_AXIS_X 0
+#define QG_AXIS_Y 1
+#define QG_AXIS_Z 2
+
+#define QG_TERM_X 0
+#define QG_TERM_Y 1
+#define QG_TERM_Z 2
+#define QG_TERM_ZZ 3
+
+/* One observable term w * P_i (kinds X, Y, Z) or w * Z_i Z_j (kind ZZ, i < j);
+ * the flattened form of the reference's observable dict
+ * (qradient/physical_components/observable.py:9-18, 44-73). */
+typedef struct qg_term {
+  int32_t kind;
+  int32_t i;
+  int32_t j;
+  int32_t pad;
+  double w;
+} qg_term;
+
+typedef struct qg_ctx qg_ctx;
+typedef struct qg_plan qg_plan;
+typedef struct qg_state qg_state;
+
+const char* qg_last_error(void);
+int qg_version(void);
+
+/* Context: one HIP device and one stream (NULL: the library creates a
+ * default-priority stream).  The stream argument is a hipStream_t. */
+int qg_ctx_create(int device, void* stream, qg_ctx** out);
+int qg_ctx_destroy(qg_ctx* ctx);
+int qg_ctx_synchronize(qg_ctx* ctx);
+
+/* ---- McClean circuit (qradient/circuit_logic/mc_clean.py) ----------------
+ * Plan for n qubits and L layers.  ckpt_every = 0 lets the planner keep the
+ * per-pass state history when it fits in HBM and otherwise checkpoint; k > 0
+ * forces checkpoints every k layers with psi uncomputed in between.
+ * Replaces the construction of McClean.__init__ (mc_clean.py:9-25). */
+int qg_mcclean_plan(qg_ctx* ctx, int n, int L, int dtype, int entangler, int ckpt_every, qg_plan** out);
+
+/* McClean.grad_run (mc_clean.py:47-78).  axes: int32 [L*n] in {0,1,2};
+ * angles: [L*n]; terms: the observable; ini_state: NULL (|0...0>) or a host
+ * complex128 vector [2*2^n] (the RY(pi/4) layer is still applied, as in the
+ * reference).  Writes E and grad[L*n] (layer-major). */
+int qg_mcclean_grad_run(qg_plan* plan, const int32_t* axes, const double* angles, const qg_term* terms,
+                        int nterms, const double* ini_state, double* E_out, double* grad_out);
+
+/* McClean.run_expec_val with exact_expec_val=True (mc_clean.py:27-45). */
+int qg_mcclean_expec_val(qg_plan* plan, const int32_t* axes, const double* angles, const qg_term* terms,
+                         int nterms, const double* ini_state, double* E_out);
+
+/* ---- QAOA circuit (qradient/circuit_logic/qaoa.py) -----------------------
+ * Plan for `batch` circuits of n qubits and p layers each. */
+int qg_qaoa_plan(qg_ctx* ctx, int n, int p, int dtype, int batch, qg_plan** out);
+
+/* Qaoa.grad_run (qaoa.py:40-70) for B <= batch circuits in one launch
+ * sequence.  Circuit b has terms[term_off[b] .. term_off[b+1]) (z / zz kinds:
+ * the classical Hamiltonian C of state.py:261-297), betas[b*p..], gammas[b*p..].
+ * grad_out[b*p*2 + i*2 + 0] = dE/dbeta_i, [.. + 1] = dE/dgamma_i. */
+int qg_qaoa_grad_run(qg_plan* plan, int B, const qg_term* terms, const int32_t* term_off, const double* betas,
+                     const double* gammas, double* E_out, double* grad_out);
+
+/* Qaoa.run_expec_val with exact_expec_val=True (qaoa.py:23-38). */
+int qg_qaoa_expec_val(qg_plan* plan, int B, const qg_term* terms, const int32_t* term_off, const double* betas,
+                      const double* gammas, double* E_out);
+
+int qg_plan_workspace_size(const qg_plan* plan, size_t* bytes);
+/* Hand the plan caller-owned device memory (e.g. a torch tensor) of at least
+ * qg_plan_workspace_size bytes; without it the plan allocates on first use. */
+int qg_plan_set_workspace(qg_plan* plan, void* dev_ptr, size_t bytes);
+/* Copy the final state of the last forward run (complex128 interleaved). */
+int qg_plan_get_state(qg_plan* plan, double* out);
+/* Device time of the last run in milliseconds (hipEvents around the launches). */
+int qg_plan_last_ms(const qg_plan* plan, double* ms);
+int qg_plan_destroy(qg_plan* plan);
+
+/* ---- device-resident State (qradient/physical_components/state.py) --------
+ * ini: '0' or '+' (state.py:61-71). */
+int qg_state_create(qg_ctx* ctx, int n, int dtype, char ini, qg_state** out);
+int qg_state_destroy(qg_state* st);
+int qg_state_reset(qg_state* st, char ini);                        /* state.py:61-71 */
+int qg_state_set(qg_state* st, const double* vec);                 /* State.vec = ...   */
+int qg_state_get(qg_state* st, double* vec);                       /* ... = State.vec   */
+/* rotation (deriv = 0) or its angle derivative (deriv = 1) about axis on qubit q:
+ * xrot/yrot/zrot state.py:90-92,142-144,168-170; dxrot/dyrot/dzrot 94-97,146-149,172-175 */
+int qg_state_rot(qg_state* st, int axis, double angle, int q, int deriv);
+int qg_state_rot_all(qg_state* st, int axis, double angle);        /* Qaoa.__xrot_all, qaoa.py:192-194 */
+int qg_state_cnot_ladder(qg_state* st, int stacking);              /* state.py:243-251 */
+int qg_state_cz_ladder(qg_state* st);
+int qg_state_cnot(qg_state* st, int control, int target);          /* state.py:198-199, 336-356 */
+/* vec <- exp(-i*angle*D) vec with D the z/zz part of terms (state.py:299-301) */
+int qg_state_exp_diag(qg_state* st, const qg_term* terms, int nterms, double angle);
+/* vec <- O vec for the full observable (multiply_matrix, mc_clean.py:65) */
+int qg_state_multiply_observable(qg_state* st, const qg_term* terms, int nterms);
+/* <vec|O|vec> in FP64 (ParametrizedCircuit.expec_val, base.py:17-20) */
+int qg_state_expec_val(qg_state* st, const qg_term* terms, int nterms, double* out);
+/* vec <- (re + i im) vec (the -1j factors of state.py:121-123, 319-321) */
+int qg_state_scale(qg_state* st, double re, double im);
+/* 1 - ||vec|| (state.py:331-332) */
+int qg_state_norm_error(qg_state* st, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QRADIENT_AMD_H */

@@ -1,0 +1,154 @@
+"""ctypes binding of libqradient_amd.so (the C ABI in include/qradient_amd.h).
+
+The library is required: importing the circuit classes works without it, but
+the first GPU call raises ``RuntimeError`` if ``libqradient_amd.so`` was not
+built (``python -m qradient_amd._build``).  There is no CPU fallback.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'libqradient_amd.so')
+
+QG_OK, QG_ERR_INVALID, QG_ERR_UNSUPPORTED, QG_ERR_HIP, QG_ERR_NOMEM = 0, 1, 2, 3, 4
+QG_C128, QG_C64 = 0, 1
+QG_ENT_CNOT, QG_ENT_CZ = 0, 1
+TERM_X, TERM_Y, TERM_Z, TERM_ZZ = 0, 1, 2, 3
+
+DTYPES = {'complex128': QG_C128, 'c128': QG_C128, np.complex128: QG_C128,
+          'complex64': QG_C64, 'c64': QG_C64, np.complex64: QG_C64}
+ENTANGLERS = {'cnot': QG_ENT_CNOT, 'cz': QG_ENT_CZ}
+
+
+class QgTerm(ctypes.Structure):
+    _fields_ = [('kind', ctypes.c_int32), ('i', ctypes.c_int32), ('j', ctypes.c_int32),
+                ('pad', ctypes.c_int32), ('w', ctypes.c_double)]
+
+
+_vp = ctypes.c_void_p
+_pp = ctypes.POINTER(ctypes.c_void_p)
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_tp = ctypes.POINTER(QgTerm)
+_i = ctypes.c_int
+_d = ctypes.c_double
+
+# name -> argtypes; every function returns int status unless listed in _RESTYPES
+SIGNATURES = {
+    'qg_last_error': [],
+    'qg_version': [],
+    'qg_ctx_create': [_i, _vp, _pp],
+    'qg_ctx_destroy': [_vp],
+    'qg_ctx_synchronize': [_vp],
+    'qg_mcclean_plan': [_vp, _i, _i, _i, _i, _i, _pp],
+    'qg_mcclean_grad_run': [_vp, _ip, _dp, _tp, _i, _dp, _dp, _dp],
+    'qg_mcclean_expec_val': [_vp, _ip, _dp, _tp, _i, _dp, _dp],
+    'qg_qaoa_plan': [_vp, _i, _i, _i, _i, _pp],
+    'qg_qaoa_grad_run': [_vp, _i, _tp, _ip, _dp, _dp, _dp, _dp],
+    'qg_qaoa_expec_val': [_vp, _i, _tp, _ip, _dp, _dp, _dp],
+    'qg_plan_workspace_size': [_vp, ctypes.POINTER(ctypes.c_size_t)],
+    'qg_plan_set_workspace': [_vp, _vp, ctypes.c_size_t],
+    'qg_plan_get_state': [_vp, _dp],
+    'qg_plan_last_ms': [_vp, _dp],
+    'qg_plan_destroy': [_vp],
+    'qg_state_create': [_vp, _i, _i, ctypes.c_char, _pp],
+    'qg_state_destroy': [_vp],
+    'qg_state_reset': [_vp, ctypes.c_char],
+    'qg_state_set': [_vp, _dp],
+    'qg_state_get': [_vp, _dp],
+    'qg_state_rot': [_vp, _i, _d, _i, _i],
+    'qg_state_rot_all': [_vp, _i, _d],
+    'qg_state_cnot_ladder': [_vp, _i],
+    'qg_state_cz_ladder': [_vp],
+    'qg_state_cnot': [_vp, _i, _i],
+    'qg_state_exp_diag': [_vp, _tp, _i, _d],
+    'qg_state_multiply_observable': [_vp, _tp, _i],
+    'qg_state_expec_val': [_vp, _tp, _i, _dp],
+    'qg_state_norm_error': [_vp, _dp],
+    'qg_state_scale': [_vp, _d, _d],
+}
+_RESTYPES = {'qg_last_error': ctypes.c_char_p}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load():
+    """Load the library (once) and declare every signature; raises if absent."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError('libqradient_amd.so is not built: run `python -m qradient_amd._build` '
+                               '(there is no CPU fallback)')
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = _RESTYPES.get(name, ctypes.c_int)
+        _lib = lib
+        return lib
+
+
+def check(status):
+    """Map a QG_* status to the reference's exception types."""
+    if status == QG_OK:
+        return
+    msg = load().qg_last_error().decode(errors='replace')
+    if status == QG_ERR_INVALID:
+        raise ValueError(msg)
+    if status == QG_ERR_UNSUPPORTED:
+        raise NotImplementedError(msg)
+    if status == QG_ERR_NOMEM:
+        raise MemoryError(msg)
+    raise RuntimeError(msg)
+
+
+def call(name, *args):
+    check(getattr(load(), name)(*args))
+
+
+def dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    return a.ctypes.data_as(_ip)
+
+
+def terms_array(terms):
+    """[(kind, i, j, w)] -> (ctypes array, count)."""
+    arr = (QgTerm * max(1, len(terms)))()
+    for k, (kind, i, j, w) in enumerate(terms):
+        arr[k].kind, arr[k].i, arr[k].j, arr[k].w = int(kind), int(i), int(j), float(w)
+    return arr, len(terms)
+
+
+class Context:
+    """One HIP device + stream; shared by the plans and states made from it."""
+
+    _default = {}
+
+    def __init__(self, device=0, stream=None):
+        h = ctypes.c_void_p()
+        call('qg_ctx_create', int(device), stream, ctypes.byref(h))
+        self.handle = h
+        self.device = device
+
+    @classmethod
+    def default(cls, device=0):
+        if device not in cls._default:
+            cls._default[device] = cls(device)
+        return cls._default[device]
+
+    def synchronize(self):
+        call('qg_ctx_synchronize', self.handle)
+
+    def __del__(self):
+        h = getattr(self, 'handle', None)
+        if h and _lib is not None:
+            _lib.qg_ctx_destroy(h)
+            self.handle = None

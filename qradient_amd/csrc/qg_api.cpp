@@ -1,0 +1,968 @@
+// Host side of the qradient_amd C ABI: contexts, plans (pass schedules,
+// workspace layout, descriptor tables), the grad_run drivers and the
+// device-resident State object.  See include/qradient_amd.h for the contract.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/qradient_amd.h"
+#include "qg_internal.h"
+
+namespace qg {
+hipError_t set_pass_attributes();
+hipError_t launch_pass(int dtype, int reg_bits, bool two, int phase, const PassDesc* d, int grid, int threads,
+                       hipStream_t st);
+hipError_t launch_finalize(int nout, const int* ptr, const int* rows, const int* cnt, const double* scale,
+                           const double* part, int stride, double* out, hipStream_t st);
+hipError_t simple_rot1(int dtype, void* v, int n, int b, int type, double c, double s, hipStream_t st);
+hipError_t simple_permute(int dtype, const void* in, void* out, int n, const uint32_t* dmap, hipStream_t st);
+hipError_t simple_diag_phase(int dtype, void* v, int n, const DTerm* dt, int nt, double angle, int cz,
+                             hipStream_t st);
+hipError_t simple_obs_apply(int dtype, const void* in, void* out, int n, const DTerm* dt, int ndt, const DTerm* xy,
+                            int nxy, hipStream_t st);
+hipError_t simple_dot(int dtype, const void* a, const void* b, int n, int mode, double* part, double* out,
+                      hipStream_t st);
+hipError_t simple_init(int dtype, void* v, int n, const double* tab, hipStream_t st);
+hipError_t simple_scale(int dtype, void* v, int n, double re, double im, hipStream_t st);
+hipError_t simple_convert_in(int dtype, const void* in_c128, void* out, int n, hipStream_t st);
+hipError_t simple_convert_out(int dtype, const void* in, void* out_c128, int n, hipStream_t st);
+constexpr int kDotBlocks = 512;
+}  // namespace qg
+
+using namespace qg;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(call)                                                                                    \
+  do {                                                                                                  \
+    hipError_t e_ = (call);                                                                             \
+    if (e_ != hipSuccess) {                                                                             \
+      return fail(e_ == hipErrorOutOfMemory ? QG_ERR_NOMEM : QG_ERR_HIP,                                \
+                  std::string(#call) + ": " + hipGetErrorString(e_));                                   \
+    }                                                                                                   \
+  } while (0)
+
+#define CHK(call)                  \
+  do {                             \
+    int r_ = (call);               \
+    if (r_ != QG_OK) return r_;    \
+  } while (0)
+
+extern "C" const char* qg_last_error(void) { return g_err.c_str(); }
+extern "C" int qg_version(void) { return 1; }
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct qg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int num_cu = 256;
+  size_t max_lds = 65536;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+extern "C" int qg_ctx_create(int device, void* stream, qg_ctx** out) {
+  if (!out) return fail(QG_ERR_INVALID, "qg_ctx_create: out is NULL");
+  int count = 0;
+  HIPCHK(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count) return fail(QG_ERR_INVALID, "qg_ctx_create: invalid device index");
+  HIPCHK(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, device));
+  if (std::string(prop.gcnArchName).rfind("gfx950", 0) != 0)
+    return fail(QG_ERR_UNSUPPORTED, std::string("qradient_amd is built for gfx950, device is ") + prop.gcnArchName);
+  qg_ctx* c = new qg_ctx();
+  c->device = device;
+  c->num_cu = prop.multiProcessorCount;
+  c->max_lds = prop.sharedMemPerBlock;
+  if (stream) {
+    c->stream = (hipStream_t)stream;
+  } else {
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete c;
+      return fail(QG_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    c->own_stream = true;
+  }
+  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+      set_pass_attributes() != hipSuccess) {
+    delete c;
+    return fail(QG_ERR_HIP, "qg_ctx_create: event creation or kernel attribute setup failed");
+  }
+  *out = c;
+  return QG_OK;
+}
+
+extern "C" int qg_ctx_destroy(qg_ctx* c) {
+  if (!c) return QG_OK;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return QG_OK;
+}
+
+extern "C" int qg_ctx_synchronize(qg_ctx* c) {
+  if (!c) return fail(QG_ERR_INVALID, "NULL context");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return QG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// observables
+// ---------------------------------------------------------------------------
+struct ObsSplit {
+  std::vector<DTerm> diag;  // z / zz with global bit positions
+  std::vector<DTerm> xy;    // a = global bit, b = 0 (X) or 1 (Y)
+};
+
+static int split_observable(int n, const qg_term* terms, int nterms, ObsSplit& o) {
+  if (nterms < 0 || (nterms > 0 && !terms)) return fail(QG_ERR_INVALID, "observable: bad term array");
+  for (int k = 0; k < nterms; ++k) {
+    const qg_term& t = terms[k];
+    if (t.i < 0 || t.i >= n) return fail(QG_ERR_INVALID, "observable: qubit index out of range");
+    if (!std::isfinite(t.w)) return fail(QG_ERR_INVALID, "observable: non-finite weight");
+    const int bi = n - 1 - t.i;
+    switch (t.kind) {
+      case QG_TERM_Z: o.diag.push_back({bi, -1, t.w}); break;
+      case QG_TERM_ZZ:
+        if (t.j <= t.i || t.j >= n)
+          return fail(QG_ERR_INVALID, "observable: zz terms need i < j < n (upper triangle, observable.py:56-64)");
+        o.diag.push_back({bi, n - 1 - t.j, t.w});
+        break;
+      case QG_TERM_X: o.xy.push_back({bi, 0, t.w}); break;
+      case QG_TERM_Y: o.xy.push_back({bi, 1, t.w}); break;
+      default: return fail(QG_ERR_INVALID, "observable: unknown term kind");
+    }
+  }
+  return QG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// pass geometry
+// ---------------------------------------------------------------------------
+struct Geo {
+  int K, t, c, p, R, threads;
+  int own_lo, own_hi;  // owned local bits [own_lo, own_hi)
+};
+
+static Geo make_geo(int K, int c, int p, int own_lo, int own_hi) {
+  Geo g;
+  g.K = K;
+  g.c = c;
+  g.p = p;
+  g.R = std::min(kRegBits, K);
+  g.t = K - g.R;
+  g.threads = 1 << g.t;
+  g.own_lo = own_lo;
+  g.own_hi = own_hi;
+  return g;
+}
+
+// Passes covering all n bits once: a low pass with local bits [0, Kl) and high
+// passes with local bits [0, c) U [p, p + K - c) that own [p, p + K - c).
+static std::vector<Geo> layer_geometry(int n, int kmax, int cmin) {
+  std::vector<Geo> g;
+  if (n <= kmax) {
+    g.push_back(make_geo(n, n, n, 0, n));
+    return g;
+  }
+  g.push_back(make_geo(kmax, kmax, n, 0, kmax));
+  const int rest = n - kmax, wmax = kmax - cmin;
+  const int h = (rest + wmax - 1) / wmax;
+  int start = kmax;
+  for (int i = 0; i < h; ++i) {
+    const int w = (rest - (start - kmax) + (h - i) - 1) / (h - i);
+    const int c = kmax - w;
+    g.push_back(make_geo(kmax, c, start, c, kmax));
+    start += w;
+  }
+  return g;
+}
+
+static int local_of_global(const Geo& g, int b) {  // -1 if not local
+  if (b < g.c) return b;
+  if (b >= g.p && b < g.p + (g.K - g.c)) return g.c + (b - g.p);
+  return -1;
+}
+
+// Columns of the basis-index map of CNOT-ladder stacking `s` (state.py:229-241):
+// stacking 1 acts the even bonds first, stacking 0 the odd bonds first.
+static void ladder_map(int n, int stacking, std::vector<uint32_t>& cols) {
+  cols.assign(n, 0);
+  for (int k = 0; k < n; ++k) {
+    uint32_t v = 1u << k;
+    for (int grp = 0; grp < 2; ++grp) {
+      const int first = (stacking == 1) ? (grp == 0 ? 0 : 1) : (grp == 0 ? 1 : 0);
+      for (int i = first; i + 1 < n; i += 2)
+        if ((v >> (n - 1 - i)) & 1u) v ^= 1u << (n - 2 - i);
+    }
+    cols[k] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// plans
+// ---------------------------------------------------------------------------
+enum PlanKind { PK_MCCLEAN = 0, PK_QAOA = 1 };
+
+struct Launch {
+  int desc;  // index into the descriptor table
+  int R, threads, grid, phase;
+  bool two;
+  int kind;  // 0 pass kernel, 1 generic observable, 2 convert-in (ini state)
+};
+
+struct qg_plan {
+  qg_ctx* ctx = nullptr;
+  int kind = PK_MCCLEAN;
+  int n = 0, L = 0, dtype = 0, ent = 0, batch = 1;
+  bool history = true;
+  int ckpt = 0;
+  int kmax = kMaxK, cmin = 2;
+  std::vector<Geo> geo;
+  size_t S = 0;  // bytes per state
+  // workspace
+  char* ws = nullptr;
+  size_t ws_bytes = 0;
+  bool ws_owned = false;
+  size_t off_hist = 0, off_lam = 0, off_ini = 0, off_prep = 0;
+  // small device tables
+  uint32_t* d_map = nullptr;    // n columns (gather/scatter)
+  double* d_init = nullptr;     // n + 1
+  DTerm* d_terms = nullptr;     // diagonal terms (max_terms)
+  DTerm* d_xy = nullptr;
+  int max_terms = 0;
+  double* d_lut = nullptr;
+  int lut_len = 0;
+  double* d_part = nullptr;
+  int gmax = 1, nrows = 0;
+  int* d_fin = nullptr;         // ptr | rows | cnt
+  double* d_scale = nullptr;
+  double* d_out = nullptr;
+  double* d_dot = nullptr;      // kDotBlocks + 1
+  PassDesc* d_desc = nullptr;
+  int ndesc_cap = 0;
+  // pinned staging
+  PassDesc* h_desc = nullptr;
+  double* h_out = nullptr;
+  int nout = 0;
+  std::vector<Launch> launches;
+  float last_ms = 0.f;
+  int last_state = -1;  // 0: final state in hist, used by get_state
+  void* last_state_ptr = nullptr;
+};
+
+static void* wsp(qg_plan* p, size_t off) { return p->ws + off; }
+
+static int ensure_workspace(qg_plan* p) {
+  if (p->ws) return QG_OK;
+  void* m = nullptr;
+  hipError_t e = hipMalloc(&m, p->ws_bytes);
+  if (e != hipSuccess)
+    return fail(QG_ERR_NOMEM, "workspace allocation of " + std::to_string(p->ws_bytes) + " bytes failed: " +
+                                  hipGetErrorString(e));
+  p->ws = (char*)m;
+  p->ws_owned = true;
+  return QG_OK;
+}
+
+static int grid_for_pass(const qg_plan* p, const Geo& g) {
+  const size_t lds = (size_t)(1u << g.K) * sizeof(double2) + (size_t)((g.threads + 63) / 64) * (kMaxK + 1) * 8;
+  int per_cu = (int)std::max<size_t>(1, (160 * 1024) / lds);
+  per_cu = std::min(per_cu, std::max(1, 2048 / g.threads));
+  const uint64_t ntiles = 1ull << (p->n - g.K);
+  uint64_t target = (uint64_t)p->ctx->num_cu * per_cu;
+  uint64_t grid = 1;
+  while (grid * 2 <= target && grid * 2 <= ntiles) grid *= 2;
+  return (int)grid;
+}
+
+static int alloc_small(qg_plan* p, int ndesc, int nrows, int nout) {
+  const int n = p->n;
+  HIPCHK(hipMalloc(&p->d_map, sizeof(uint32_t) * std::max(n, 1)));
+  HIPCHK(hipMalloc(&p->d_init, sizeof(double) * (n + 1)));
+  p->max_terms = std::max(64, n * n);
+  HIPCHK(hipMalloc(&p->d_terms, sizeof(DTerm) * p->max_terms));
+  HIPCHK(hipMalloc(&p->d_xy, sizeof(DTerm) * p->max_terms));
+  HIPCHK(hipMalloc(&p->d_part, sizeof(double) * (size_t)std::max(1, nrows) * p->gmax));
+  HIPCHK(hipMemset(p->d_part, 0, sizeof(double) * (size_t)std::max(1, nrows) * p->gmax));
+  HIPCHK(hipMalloc(&p->d_fin, sizeof(int) * (size_t)(3 * (nout + 1) + 3 * nrows + 8)));
+  HIPCHK(hipMalloc(&p->d_scale, sizeof(double) * std::max(1, nout)));
+  HIPCHK(hipMalloc(&p->d_out, sizeof(double) * std::max(1, nout)));
+  HIPCHK(hipMalloc(&p->d_dot, sizeof(double) * (kDotBlocks + 1)));
+  HIPCHK(hipMalloc(&p->d_desc, sizeof(PassDesc) * std::max(1, ndesc)));
+  HIPCHK(hipHostMalloc(&p->h_desc, sizeof(PassDesc) * std::max(1, ndesc), hipHostMallocDefault));
+  HIPCHK(hipHostMalloc(&p->h_out, sizeof(double) * std::max(1, nout), hipHostMallocDefault));
+  p->ndesc_cap = ndesc;
+  p->nout = nout;
+  return QG_OK;
+}
+
+extern "C" int qg_plan_destroy(qg_plan* p) {
+  if (!p) return QG_OK;
+  hipSetDevice(p->ctx->device);
+  hipStreamSynchronize(p->ctx->stream);
+  if (p->ws_owned && p->ws) hipFree(p->ws);
+  hipFree(p->d_map);
+  hipFree(p->d_init);
+  hipFree(p->d_terms);
+  hipFree(p->d_xy);
+  hipFree(p->d_lut);
+  hipFree(p->d_part);
+  hipFree(p->d_fin);
+  hipFree(p->d_scale);
+  hipFree(p->d_out);
+  hipFree(p->d_dot);
+  hipFree(p->d_desc);
+  if (p->h_desc) hipHostFree(p->h_desc);
+  if (p->h_out) hipHostFree(p->h_out);
+  delete p;
+  return QG_OK;
+}
+
+extern "C" int qg_plan_workspace_size(const qg_plan* p, size_t* bytes) {
+  if (!p || !bytes) return fail(QG_ERR_INVALID, "NULL argument");
+  *bytes = p->ws_bytes;
+  return QG_OK;
+}
+
+extern "C" int qg_plan_set_workspace(qg_plan* p, void* dev, size_t bytes) {
+  if (!p || !dev) return fail(QG_ERR_INVALID, "NULL argument");
+  if (bytes < p->ws_bytes) return fail(QG_ERR_INVALID, "workspace smaller than qg_plan_workspace_size");
+  if (((uintptr_t)dev) % 256 != 0) return fail(QG_ERR_INVALID, "workspace must be 256-byte aligned");
+  if (p->ws_owned && p->ws) {
+    hipStreamSynchronize(p->ctx->stream);
+    hipFree(p->ws);
+  }
+  p->ws = (char*)dev;
+  p->ws_owned = false;
+  return QG_OK;
+}
+
+extern "C" int qg_plan_last_ms(const qg_plan* p, double* ms) {
+  if (!p || !ms) return fail(QG_ERR_INVALID, "NULL argument");
+  *ms = p->last_ms;
+  return QG_OK;
+}
+
+extern "C" int qg_plan_get_state(qg_plan* p, double* out) {
+  if (!p || !out) return fail(QG_ERR_INVALID, "NULL argument");
+  if (!p->last_state_ptr) return fail(QG_ERR_INVALID, "no state: run the circuit first");
+  HIPCHK(hipSetDevice(p->ctx->device));
+  const size_t N = 1ull << p->n;
+  double* tmp = nullptr;
+  HIPCHK(hipMalloc(&tmp, N * 16));
+  hipError_t e = simple_convert_out(p->dtype, p->last_state_ptr, tmp, p->n, p->ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, tmp, N * 16, hipMemcpyDeviceToHost, p->ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(p->ctx->stream);
+  hipFree(tmp);
+  if (e != hipSuccess) return fail(QG_ERR_HIP, std::string("qg_plan_get_state: ") + hipGetErrorString(e));
+  return QG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// descriptor helpers
+// ---------------------------------------------------------------------------
+static void desc_geo(PassDesc& d, int n, const Geo& g) {
+  std::memset(&d, 0, sizeof(PassDesc));
+  d.n = n;
+  d.K = g.K;
+  d.t = g.t;
+  d.c = g.c;
+  d.p = g.p;
+  d.row_base = -1;
+  for (int k = 0; k < kMaxK; ++k) {
+    d.gtype[k] = G_NONE;
+    d.gc[k] = 1.0;
+    d.gs[k] = 0.0;
+  }
+}
+
+static void set_gate(PassDesc& d, int lb, int axis, double angle, double sign) {
+  d.gtype[lb] = axis == 0 ? G_X : axis == 1 ? G_Y : G_Z;
+  d.gc[lb] = std::cos(0.5 * angle);
+  d.gs[lb] = sign * std::sin(0.5 * angle);
+}
+
+static int run_launches(qg_plan* p, int ndesc) {
+  qg_ctx* c = p->ctx;
+  HIPCHK(hipMemcpyAsync(p->d_desc, p->h_desc, sizeof(PassDesc) * ndesc, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  for (const Launch& l : p->launches) {
+    if (l.kind == 0) {
+      HIPCHK(launch_pass(p->dtype, l.R, l.two, l.phase, p->d_desc + l.desc, l.grid, l.threads, c->stream));
+    }
+  }
+  return QG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// McClean
+// ---------------------------------------------------------------------------
+extern "C" int qg_mcclean_plan(qg_ctx* ctx, int n, int L, int dtype, int entangler, int ckpt_every,
+                               qg_plan** out) {
+  if (!ctx || !out) return fail(QG_ERR_INVALID, "NULL argument");
+  if (n < 1 || n > kMaxQubits) return fail(QG_ERR_INVALID, "qubit number must be in [1, 32]");
+  if (L < 1) return fail(QG_ERR_INVALID, "layer number must be >= 1");
+  if (dtype != QG_C128 && dtype != QG_C64) return fail(QG_ERR_INVALID, "dtype must be QG_C128 or QG_C64");
+  if (entangler != QG_ENT_CNOT && entangler != QG_ENT_CZ) return fail(QG_ERR_INVALID, "unknown entangler");
+  if (ckpt_every < 0) return fail(QG_ERR_INVALID, "ckpt_every must be >= 0");
+  HIPCHK(hipSetDevice(ctx->device));
+  qg_plan* p = new qg_plan();
+  p->ctx = ctx;
+  p->kind = PK_MCCLEAN;
+  p->n = n;
+  p->L = L;
+  p->dtype = dtype;
+  p->ent = entangler;
+  p->geo = layer_geometry(n, p->kmax, p->cmin);
+  const int P = (int)p->geo.size();
+  p->S = (size_t)(1ull << n) * (dtype == QG_C64 ? 8 : 16);
+  size_t free_b = 0, total_b = 0;
+  hipMemGetInfo(&free_b, &total_b);
+  const size_t hist_bytes = (size_t)(P * L + 4) * p->S;
+  if (ckpt_every > 0 || hist_bytes > (size_t)(0.85 * (double)free_b)) {
+    delete p;
+    return fail(QG_ERR_UNSUPPORTED, "checkpointed (uncompute) mode is not available in this build for this size");
+  }
+  p->history = true;
+  p->off_hist = 0;
+  p->off_lam = (size_t)P * L * p->S;
+  p->off_ini = p->off_lam + 2 * p->S;
+  p->off_prep = p->off_ini + p->S;
+  p->ws_bytes = p->off_prep + p->S;
+  for (const Geo& g : p->geo) p->gmax = std::max(p->gmax, grid_for_pass(p, g));
+  const int npass = 2 * P * L + 2 * P;  // forward + reverse + ini-state preparation
+  p->nrows = (P * L + 1) * (kMaxK + 1);
+  int r = alloc_small(p, npass, p->nrows, 1 + L * n);
+  if (r != QG_OK) {
+    qg_plan_destroy(p);
+    return r;
+  }
+  // static tables
+  std::vector<uint32_t> map;
+  ladder_map(n, 1, map);
+  std::vector<double> init(n + 1);
+  const double c8 = std::cos(M_PI / 8), s8 = std::sin(M_PI / 8);
+  for (int k = 0; k <= n; ++k) init[k] = std::pow(c8, n - k) * std::pow(s8, k);
+  hipError_t e = hipMemcpy(p->d_map, map.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->d_init, init.data(), sizeof(double) * (n + 1), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    qg_plan_destroy(p);
+    return fail(QG_ERR_HIP, std::string("plan tables: ") + hipGetErrorString(e));
+  }
+  *out = p;
+  return QG_OK;
+}
+
+// Builds the pass list of one McClean run into p->h_desc / p->launches and
+// the finalize tables; returns the descriptor count.
+static int mcclean_schedule(qg_plan* p, const int32_t* axes, const double* angles, const ObsSplit& obs,
+                            bool with_ini, bool grad, int& ndesc_out) {
+  const int n = p->n, L = p->L, P = (int)p->geo.size();
+  const bool diag_obs = obs.xy.empty();
+  p->launches.clear();
+  int nd = 0;
+  int row = 0;
+  std::vector<int> fin_rows(1 + L * n, -1);
+  std::vector<int> row_cnt(p->nrows, 0);
+  auto hist = [&](int i, int g) { return wsp(p, p->off_hist + ((size_t)i * P + g) * p->S); };
+  void* lam[2] = {wsp(p, p->off_lam), wsp(p, p->off_lam + p->S)};
+  const uint32_t ent_flag_fwd = p->ent == QG_ENT_CNOT ? F_GATHER : F_CZ_PRE;
+  const uint32_t ent_flag_rev = p->ent == QG_ENT_CNOT ? F_SCATTER : F_CZ_POST;
+
+  auto push = [&](const Geo& g, bool two) -> PassDesc& {
+    Launch l;
+    l.desc = nd;
+    l.R = g.R;
+    l.threads = g.threads;
+    l.grid = grid_for_pass(p, g);
+    l.phase = 0;
+    l.two = two;
+    l.kind = 0;
+    p->launches.push_back(l);
+    PassDesc& d = p->h_desc[nd++];
+    desc_geo(d, n, g);
+    d.grid = p->gmax;  // stride of the partial rows; the launch grid is l.grid
+    d.map = p->d_map;
+    d.init_tab = p->d_init;
+    d.dterms = p->d_terms;
+    d.ndterms = (int)obs.diag.size();
+    d.part = p->d_part;
+    return d;
+  };
+
+  // optional ini_state preparation: RY(pi/4) on every qubit of the given vector
+  const void* prev = nullptr;
+  if (with_ini) {
+    const void* src = wsp(p, p->off_ini);
+    for (int g = 0; g < P; ++g) {
+      const Geo& G = p->geo[g];
+      PassDesc& d = push(G, false);
+      d.flags = F_LOAD_A | F_ROT_A | F_STORE_A;
+      d.a_in = src;
+      d.a_out = wsp(p, p->off_prep);
+      for (int lb = G.own_lo; lb < G.own_hi; ++lb) set_gate(d, lb, QG_AXIS_Y, M_PI / 4, 1.0);
+      src = wsp(p, p->off_prep);
+    }
+    prev = wsp(p, p->off_prep);
+  }
+  // forward
+  for (int i = 0; i < L; ++i) {
+    for (int g = 0; g < P; ++g) {
+      const Geo& G = p->geo[g];
+      const bool last = (i == L - 1) && (g == P - 1);
+      const bool obs_fused = last && diag_obs;
+      PassDesc& d = push(G, obs_fused);
+      uint32_t f = F_ROT_A | F_STORE_A;
+      if (g == 0) {
+        f |= ent_flag_fwd;
+        f |= prev ? F_LOAD_A : F_INIT_TAB;
+      } else {
+        f |= F_LOAD_A;
+      }
+      if (obs_fused) {
+        f |= F_OBS_DIAG | F_STORE_B;
+        d.b_out = lam[0];
+        d.row_base = row;
+        fin_rows[0] = row + G.K;
+        row += kMaxK + 1;
+      }
+      d.flags = f;
+      d.a_in = prev;
+      d.a_out = hist(i, g);
+      for (int lb = G.own_lo; lb < G.own_hi; ++lb) {
+        const int b = (lb < G.c) ? lb : G.p + (lb - G.c);
+        const int q = n - 1 - b;
+        set_gate(d, lb, axes[i * n + q], angles[i * n + q], 1.0);
+      }
+      prev = hist(i, g);
+    }
+  }
+  p->last_state_ptr = hist(L - 1, P - 1);
+  if (!grad) {
+    ndesc_out = nd;
+    std::vector<int> ptr(2), rows;
+    if (fin_rows[0] >= 0) rows.push_back(fin_rows[0]);
+    ptr[0] = 0;
+    ptr[1] = (int)rows.size();
+    (void)ptr;
+    return QG_OK;
+  }
+  // reverse
+  int cur = 0;
+  for (int i = L - 1; i >= 0; --i) {
+    for (int g = P - 1; g >= 0; --g) {
+      const Geo& G = p->geo[g];
+      PassDesc& d = push(G, true);
+      uint32_t f = F_LOAD_A | F_ROT_A | F_LOAD_B | F_ROT_B | F_STORE_B | F_REDUCE;
+      d.a_in = hist(i, g);
+      d.b_in = lam[cur];
+      if (g == 0) {
+        f |= ent_flag_rev;
+        if (p->ent == QG_ENT_CNOT) {
+          d.b_out = lam[cur ^ 1];
+          cur ^= 1;
+        } else {
+          d.b_out = lam[cur];
+        }
+      } else {
+        d.b_out = lam[cur];
+      }
+      d.flags = f;
+      d.row_base = row;
+      for (int lb = G.own_lo; lb < G.own_hi; ++lb) {
+        const int b = (lb < G.c) ? lb : G.p + (lb - G.c);
+        const int q = n - 1 - b;
+        set_gate(d, lb, axes[i * n + q], angles[i * n + q], -1.0);
+        d.own[lb] = 1;
+        fin_rows[1 + i * n + q] = row + lb;
+      }
+      row += kMaxK + 1;
+    }
+  }
+  ndesc_out = nd;
+  (void)row_cnt;
+  return QG_OK;
+}
+
+static int upload_finalize(qg_plan* p, const std::vector<std::vector<int>>& out_rows, const std::vector<int>& cnt,
+                           const std::vector<double>& scale) {
+  const int nout = (int)out_rows.size();
+  std::vector<int> buf;
+  buf.push_back(0);
+  for (int i = 0; i < nout; ++i) buf.push_back(buf.back() + (int)out_rows[i].size());
+  for (int i = 0; i < nout; ++i)
+    for (int r : out_rows[i]) buf.push_back(r);
+  const int rows_off = nout + 1;
+  const int cnt_off = (int)buf.size();
+  for (int r = 0; r < p->nrows; ++r) buf.push_back(r < (int)cnt.size() ? cnt[r] : 0);
+  HIPCHK(hipMemcpyAsync(p->d_fin, buf.data(), sizeof(int) * buf.size(), hipMemcpyHostToDevice, p->ctx->stream));
+  HIPCHK(hipMemcpyAsync(p->d_scale, scale.data(), sizeof(double) * nout, hipMemcpyHostToDevice, p->ctx->stream));
+  HIPCHK(launch_finalize(nout, p->d_fin, p->d_fin + rows_off, p->d_fin + cnt_off, p->d_scale, p->d_part, p->gmax,
+                         p->d_out, p->ctx->stream));
+  // synchronous because the host vectors above die at return
+  HIPCHK(hipStreamSynchronize(p->ctx->stream));
+  return QG_OK;
+}
+
+static int mcclean_run(qg_plan* p, const int32_t* axes, const double* angles, const qg_term* terms, int nterms,
+                       const double* ini_state, double* E_out, double* grad_out, bool grad) {
+  if (!p || p->kind != PK_MCCLEAN) return fail(QG_ERR_INVALID, "not a McClean plan");
+  if (!axes || !angles || !E_out || (grad && !grad_out)) return fail(QG_ERR_INVALID, "NULL argument");
+  const int n = p->n, L = p->L;
+  for (int k = 0; k < L * n; ++k) {
+    if (axes[k] < 0 || axes[k] > 2) return fail(QG_ERR_INVALID, "Invalid axis " + std::to_string(axes[k]));
+    if (!std::isfinite(angles[k])) return fail(QG_ERR_INVALID, "non-finite angle");
+  }
+  ObsSplit obs;
+  CHK(split_observable(n, terms, nterms, obs));
+  if ((int)obs.diag.size() > p->max_terms || (int)obs.xy.size() > p->max_terms)
+    return fail(QG_ERR_INVALID, "too many observable terms");
+  if (!obs.xy.empty())
+    return fail(QG_ERR_UNSUPPORTED, "observables with x/y terms are not supported by grad_run in this build");
+  qg_ctx* c = p->ctx;
+  HIPCHK(hipSetDevice(c->device));
+  CHK(ensure_workspace(p));
+  if (!obs.diag.empty())
+    HIPCHK(hipMemcpyAsync(p->d_terms, obs.diag.data(), sizeof(DTerm) * obs.diag.size(), hipMemcpyHostToDevice,
+                          c->stream));
+  if (ini_state) {
+    const size_t N = 1ull << n;
+    double* tmp = nullptr;
+    HIPCHK(hipMalloc(&tmp, N * 16));
+    hipError_t e = hipMemcpyAsync(tmp, ini_state, N * 16, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = simple_convert_in(p->dtype, tmp, wsp(p, p->off_ini), n, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(tmp);
+    if (e != hipSuccess) return fail(QG_ERR_HIP, std::string("ini_state upload: ") + hipGetErrorString(e));
+  }
+  int nd = 0;
+  CHK(mcclean_schedule(p, axes, angles, obs, ini_state != nullptr, grad, nd));
+  // rows written by each pass and the finalize table
+  const int P = (int)p->geo.size();
+  std::vector<std::vector<int>> out_rows(grad ? 1 + L * n : 1);
+  std::vector<int> cnt(p->nrows, 0);
+  for (const Launch& l : p->launches) {
+    const PassDesc& d = p->h_desc[l.desc];
+    if (d.row_base < 0) continue;
+    for (int k = 0; k <= kMaxK; ++k) cnt[d.row_base + k] = l.grid;
+  }
+  {
+    // E: the row K of the fused observable pass (the last forward pass)
+    int fwd_last = (ini_state ? P : 0) + P * L - 1;
+    const PassDesc& d = p->h_desc[fwd_last];
+    out_rows[0].push_back(d.row_base + d.K);
+    if (grad) {
+      int idx = fwd_last + 1;
+      for (int i = L - 1; i >= 0; --i) {
+        for (int g = P - 1; g >= 0; --g, ++idx) {
+          const PassDesc& dr = p->h_desc[idx];
+          const Geo& G = p->geo[g];
+          for (int lb = G.own_lo; lb < G.own_hi; ++lb) {
+            const int b = (lb < G.c) ? lb : G.p + (lb - G.c);
+            const int q = n - 1 - b;
+            out_rows[1 + i * n + q].push_back(dr.row_base + lb);
+          }
+        }
+      }
+    }
+  }
+  std::vector<double> scale(out_rows.size(), 1.0);
+  CHK(run_launches(p, nd));
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  CHK(upload_finalize(p, out_rows, cnt, scale));
+  HIPCHK(hipEventElapsedTime(&p->last_ms, c->ev0, c->ev1));
+  const int nout = (int)out_rows.size();
+  HIPCHK(hipMemcpyAsync(p->h_out, p->d_out, sizeof(double) * nout, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *E_out = p->h_out[0];
+  if (grad) std::memcpy(grad_out, p->h_out + 1, sizeof(double) * L * n);
+  return QG_OK;
+}
+
+extern "C" int qg_mcclean_grad_run(qg_plan* p, const int32_t* axes, const double* angles, const qg_term* terms,
+                                   int nterms, const double* ini_state, double* E_out, double* grad_out) {
+  return mcclean_run(p, axes, angles, terms, nterms, ini_state, E_out, grad_out, true);
+}
+
+extern "C" int qg_mcclean_expec_val(qg_plan* p, const int32_t* axes, const double* angles, const qg_term* terms,
+                                    int nterms, const double* ini_state, double* E_out) {
+  return mcclean_run(p, axes, angles, terms, nterms, ini_state, E_out, nullptr, false);
+}
+
+// ---------------------------------------------------------------------------
+// QAOA (not yet available)
+// ---------------------------------------------------------------------------
+extern "C" int qg_qaoa_plan(qg_ctx*, int, int, int, int, qg_plan**) {
+  return fail(QG_ERR_UNSUPPORTED, "QAOA plans are not available in this build");
+}
+extern "C" int qg_qaoa_grad_run(qg_plan*, int, const qg_term*, const int32_t*, const double*, const double*, double*,
+                                double*) {
+  return fail(QG_ERR_UNSUPPORTED, "QAOA plans are not available in this build");
+}
+extern "C" int qg_qaoa_expec_val(qg_plan*, int, const qg_term*, const int32_t*, const double*, const double*,
+                                 double*) {
+  return fail(QG_ERR_UNSUPPORTED, "QAOA plans are not available in this build");
+}
+
+// ---------------------------------------------------------------------------
+// device-resident State
+// ---------------------------------------------------------------------------
+struct qg_state {
+  qg_ctx* ctx = nullptr;
+  int n = 0, dtype = 0;
+  void* v = nullptr;
+  void* tmp = nullptr;
+  uint32_t* d_map = nullptr;
+  DTerm* d_terms = nullptr;
+  DTerm* d_xy = nullptr;
+  int max_terms = 0;
+  double* d_tab = nullptr;
+  double* d_dot = nullptr;
+};
+
+static int state_init(qg_state* s, char ini) {
+  const int n = s->n;
+  std::vector<double> tab(n + 1, 0.0);
+  if (ini == '0') {
+    tab[0] = 1.0;
+  } else if (ini == '+') {
+    for (int k = 0; k <= n; ++k) tab[k] = std::pow(2.0, -0.5 * n);
+  } else {
+    return fail(QG_ERR_INVALID, std::string("Invalid initialization format ") + ini + ".");
+  }
+  HIPCHK(hipMemcpyAsync(s->d_tab, tab.data(), sizeof(double) * (n + 1), hipMemcpyHostToDevice, s->ctx->stream));
+  HIPCHK(simple_init(s->dtype, s->v, n, s->d_tab, s->ctx->stream));
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  return QG_OK;
+}
+
+extern "C" int qg_state_destroy(qg_state* s) {
+  if (!s) return QG_OK;
+  hipSetDevice(s->ctx->device);
+  hipStreamSynchronize(s->ctx->stream);
+  hipFree(s->v);
+  hipFree(s->tmp);
+  hipFree(s->d_map);
+  hipFree(s->d_terms);
+  hipFree(s->d_xy);
+  hipFree(s->d_tab);
+  hipFree(s->d_dot);
+  delete s;
+  return QG_OK;
+}
+
+extern "C" int qg_state_create(qg_ctx* ctx, int n, int dtype, char ini, qg_state** out) {
+  if (!ctx || !out) return fail(QG_ERR_INVALID, "NULL argument");
+  if (n < 1 || n > kMaxQubits) return fail(QG_ERR_INVALID, "qubit number must be in [1, 32]");
+  if (dtype != QG_C128 && dtype != QG_C64) return fail(QG_ERR_INVALID, "bad dtype");
+  if (ini != '0' && ini != '+') return fail(QG_ERR_INVALID, std::string("Invalid initialization format ") + ini + ".");
+  HIPCHK(hipSetDevice(ctx->device));
+  qg_state* s = new qg_state();
+  s->ctx = ctx;
+  s->n = n;
+  s->dtype = dtype;
+  s->max_terms = std::max(64, n * n);
+  const size_t bytes = (size_t)(1ull << n) * (dtype == QG_C64 ? 8 : 16);
+  hipError_t e = hipMalloc(&s->v, bytes);
+  if (e == hipSuccess) e = hipMalloc(&s->tmp, bytes);
+  if (e == hipSuccess) e = hipMalloc(&s->d_map, sizeof(uint32_t) * n);
+  if (e == hipSuccess) e = hipMalloc(&s->d_terms, sizeof(DTerm) * s->max_terms);
+  if (e == hipSuccess) e = hipMalloc(&s->d_xy, sizeof(DTerm) * s->max_terms);
+  if (e == hipSuccess) e = hipMalloc(&s->d_tab, sizeof(double) * (n + 1));
+  if (e == hipSuccess) e = hipMalloc(&s->d_dot, sizeof(double) * (kDotBlocks + 1));
+  if (e != hipSuccess) {
+    qg_state_destroy(s);
+    return fail(e == hipErrorOutOfMemory ? QG_ERR_NOMEM : QG_ERR_HIP, std::string("qg_state_create: ") + hipGetErrorString(e));
+  }
+  int r = state_init(s, ini);
+  if (r != QG_OK) {
+    qg_state_destroy(s);
+    return r;
+  }
+  *out = s;
+  return QG_OK;
+}
+
+extern "C" int qg_state_reset(qg_state* s, char ini) {
+  if (!s) return fail(QG_ERR_INVALID, "NULL state");
+  HIPCHK(hipSetDevice(s->ctx->device));
+  return state_init(s, ini);
+}
+
+extern "C" int qg_state_set(qg_state* s, const double* vec) {
+  if (!s || !vec) return fail(QG_ERR_INVALID, "NULL argument");
+  HIPCHK(hipSetDevice(s->ctx->device));
+  const size_t N = 1ull << s->n;
+  double* tmp = nullptr;
+  HIPCHK(hipMalloc(&tmp, N * 16));
+  hipError_t e = hipMemcpyAsync(tmp, vec, N * 16, hipMemcpyHostToDevice, s->ctx->stream);
+  if (e == hipSuccess) e = simple_convert_in(s->dtype, tmp, s->v, s->n, s->ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->ctx->stream);
+  hipFree(tmp);
+  if (e != hipSuccess) return fail(QG_ERR_HIP, std::string("qg_state_set: ") + hipGetErrorString(e));
+  return QG_OK;
+}
+
+extern "C" int qg_state_get(qg_state* s, double* vec) {
+  if (!s || !vec) return fail(QG_ERR_INVALID, "NULL argument");
+  HIPCHK(hipSetDevice(s->ctx->device));
+  const size_t N = 1ull << s->n;
+  double* tmp = nullptr;
+  HIPCHK(hipMalloc(&tmp, N * 16));
+  hipError_t e = simple_convert_out(s->dtype, s->v, tmp, s->n, s->ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(vec, tmp, N * 16, hipMemcpyDeviceToHost, s->ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->ctx->stream);
+  hipFree(tmp);
+  if (e != hipSuccess) return fail(QG_ERR_HIP, std::string("qg_state_get: ") + hipGetErrorString(e));
+  return QG_OK;
+}
+
+extern "C" int qg_state_rot(qg_state* s, int axis, double angle, int q, int deriv) {
+  if (!s) return fail(QG_ERR_INVALID, "NULL state");
+  if (axis < 0 || axis > 2) return fail(QG_ERR_INVALID, "Invalid axis " + std::to_string(axis));
+  if (q < 0 || q >= s->n) return fail(QG_ERR_INVALID, "qubit index out of range");
+  HIPCHK(hipSetDevice(s->ctx->device));
+  const int type = axis == 0 ? G_X : axis == 1 ? G_Y : G_Z;
+  double c = std::cos(0.5 * angle), sn = std::sin(0.5 * angle);
+  if (deriv) {  // d/da (cos(a/2) I - i sin(a/2) P) = -sin(a/2)/2 I - i cos(a/2)/2 P
+    const double c2 = -0.5 * sn, s2 = 0.5 * c;
+    c = c2;
+    sn = s2;
+  }
+  HIPCHK(simple_rot1(s->dtype, s->v, s->n, s->n - 1 - q, type, c, sn, s->ctx->stream));
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  return QG_OK;
+}
+
+extern "C" int qg_state_rot_all(qg_state* s, int axis, double angle) {
+  if (!s) return fail(QG_ERR_INVALID, "NULL state");
+  for (int q = 0; q < s->n; ++q) CHK(qg_state_rot(s, axis, angle, q, 0));
+  return QG_OK;
+}
+
+static int state_permute(qg_state* s, const std::vector<uint32_t>& cols) {
+  HIPCHK(hipMemcpyAsync(s->d_map, cols.data(), sizeof(uint32_t) * s->n, hipMemcpyHostToDevice, s->ctx->stream));
+  HIPCHK(simple_permute(s->dtype, s->v, s->tmp, s->n, s->d_map, s->ctx->stream));
+  std::swap(s->v, s->tmp);
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  return QG_OK;
+}
+
+extern "C" int qg_state_cnot_ladder(qg_state* s, int stacking) {
+  if (!s) return fail(QG_ERR_INVALID, "NULL state");
+  if (stacking != 0 && stacking != 1) return fail(QG_ERR_INVALID, "stacking must be 0 or 1");
+  HIPCHK(hipSetDevice(s->ctx->device));
+  // gather: (L psi)(y) = psi(L^{-1} y); L0^{-1} = L1 and L1^{-1} = L0
+  std::vector<uint32_t> cols;
+  ladder_map(s->n, stacking == 0 ? 1 : 0, cols);
+  return state_permute(s, cols);
+}
+
+extern "C" int qg_state_cnot(qg_state* s, int control, int target) {
+  if (!s) return fail(QG_ERR_INVALID, "NULL state");
+  const int n = s->n;
+  if (control < 0 || target < 0 || control >= n || target >= n || control == target)
+    return fail(QG_ERR_INVALID, "Invalid CNOT indecies " + std::to_string(control) + " and " +
+                                    std::to_string(target) + ", for " + std::to_string(n) + " qubits.");
+  HIPCHK(hipSetDevice(s->ctx->device));
+  std::vector<uint32_t> cols(n);
+  const int bc = n - 1 - control, bt = n - 1 - target;
+  for (int k = 0; k < n; ++k) cols[k] = (1u << k) | (k == bc ? (1u << bt) : 0u);
+  return state_permute(s, cols);
+}
+
+extern "C" int qg_state_cz_ladder(qg_state* s) {
+  if (!s) return fail(QG_ERR_INVALID, "NULL state");
+  HIPCHK(hipSetDevice(s->ctx->device));
+  HIPCHK(simple_diag_phase(s->dtype, s->v, s->n, s->d_terms, 0, 0.0, 1, s->ctx->stream));
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  return QG_OK;
+}
+
+static int state_upload_obs(qg_state* s, const qg_term* terms, int nterms, ObsSplit& o) {
+  CHK(split_observable(s->n, terms, nterms, o));
+  if ((int)o.diag.size() > s->max_terms || (int)o.xy.size() > s->max_terms)
+    return fail(QG_ERR_INVALID, "too many observable terms");
+  if (!o.diag.empty())
+    HIPCHK(hipMemcpyAsync(s->d_terms, o.diag.data(), sizeof(DTerm) * o.diag.size(), hipMemcpyHostToDevice,
+                          s->ctx->stream));
+  if (!o.xy.empty())
+    HIPCHK(hipMemcpyAsync(s->d_xy, o.xy.data(), sizeof(DTerm) * o.xy.size(), hipMemcpyHostToDevice, s->ctx->stream));
+  return QG_OK;
+}
+
+extern "C" int qg_state_exp_diag(qg_state* s, const qg_term* terms, int nterms, double angle) {
+  if (!s) return fail(QG_ERR_INVALID, "NULL state");
+  HIPCHK(hipSetDevice(s->ctx->device));
+  ObsSplit o;
+  CHK(state_upload_obs(s, terms, nterms, o));
+  HIPCHK(simple_diag_phase(s->dtype, s->v, s->n, s->d_terms, (int)o.diag.size(), angle, 0, s->ctx->stream));
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  return QG_OK;
+}
+
+extern "C" int qg_state_multiply_observable(qg_state* s, const qg_term* terms, int nterms) {
+  if (!s) return fail(QG_ERR_INVALID, "NULL state");
+  HIPCHK(hipSetDevice(s->ctx->device));
+  ObsSplit o;
+  CHK(state_upload_obs(s, terms, nterms, o));
+  HIPCHK(simple_obs_apply(s->dtype, s->v, s->tmp, s->n, s->d_terms, (int)o.diag.size(), s->d_xy, (int)o.xy.size(),
+                          s->ctx->stream));
+  std::swap(s->v, s->tmp);
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  return QG_OK;
+}
+
+extern "C" int qg_state_expec_val(qg_state* s, const qg_term* terms, int nterms, double* out) {
+  if (!s || !out) return fail(QG_ERR_INVALID, "NULL argument");
+  HIPCHK(hipSetDevice(s->ctx->device));
+  ObsSplit o;
+  CHK(state_upload_obs(s, terms, nterms, o));
+  HIPCHK(simple_obs_apply(s->dtype, s->v, s->tmp, s->n, s->d_terms, (int)o.diag.size(), s->d_xy, (int)o.xy.size(),
+                          s->ctx->stream));
+  HIPCHK(simple_dot(s->dtype, s->v, s->tmp, s->n, 0, s->d_dot, s->d_dot + kDotBlocks, s->ctx->stream));
+  HIPCHK(hipMemcpyAsync(out, s->d_dot + kDotBlocks, sizeof(double), hipMemcpyDeviceToHost, s->ctx->stream));
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  return QG_OK;
+}
+
+extern "C" int qg_state_norm_error(qg_state* s, double* out) {
+  if (!s || !out) return fail(QG_ERR_INVALID, "NULL argument");
+  HIPCHK(hipSetDevice(s->ctx->device));
+  double nrm2 = 0.0;
+  HIPCHK(simple_dot(s->dtype, s->v, s->v, s->n, 1, s->d_dot, s->d_dot + kDotBlocks, s->ctx->stream));
+  HIPCHK(hipMemcpyAsync(&nrm2, s->d_dot + kDotBlocks, sizeof(double), hipMemcpyDeviceToHost, s->ctx->stream));
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  *out = 1.0 - std::sqrt(nrm2);
+  return QG_OK;
+}
+
+extern "C" int qg_state_scale(qg_state* s, double re, double im) {
+  if (!s) return fail(QG_ERR_INVALID, "NULL state");
+  HIPCHK(hipSetDevice(s->ctx->device));
+  HIPCHK(simple_scale(s->dtype, s->v, s->n, re, im, s->ctx->stream));
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  return QG_OK;
+}

@@ -1,0 +1,71 @@
+// Internal descriptors shared by the host planner (qg_plan.cpp) and the HIP
+// kernels (qg_kernels.hip).  Plain-old-data only: the planner writes arrays of
+// these structs into device memory and every kernel reads them through a
+// pointer, so a captured hipGraph stays valid while the per-call contents
+// (angles, observable weights) change.
+#pragma once
+#include <stdint.h>
+
+namespace qg {
+
+constexpr int kMaxK = 13;        // max local (tile) bits of a pass: 2^13 FP64 complex = 128 KiB LDS
+constexpr int kRegBits = 4;      // amplitudes held per thread = 2^kRegBits
+constexpr int kMaxStages = 4;    // natural stage + ceil((kMaxK - kRegBits) / kRegBits)
+constexpr int kMaxQubits = 32;     // 32-bit amplitude indices
+
+enum GateType : int32_t { G_NONE = 0, G_X = 1, G_Y = 2, G_Z = 3 };
+
+// Pass flags.
+enum : uint32_t {
+  F_LOAD_A      = 1u << 0,   // A is read from a_in
+  F_INIT_TAB    = 1u << 1,   // A is generated: amp(x) = init_tab[popcount(x)]
+  F_LOAD_B      = 1u << 2,   // B (the adjoint state lambda) is read from b_in
+  F_ROT_A       = 1u << 3,   // apply the pass gates to A
+  F_ROT_B       = 1u << 4,   // apply the pass gates to B
+  F_STORE_A     = 1u << 5,
+  F_STORE_B     = 1u << 6,
+  F_REDUCE      = 1u << 7,   // per owned bit: Im<B|P_q|A> before the gates
+  F_GATHER      = 1u << 8,   // load address = map(x) (ladder gather, forward)
+  F_SCATTER     = 1u << 9,   // store address = map(x) (ladder scatter, reverse)
+  F_CZ_PRE      = 1u << 10,  // multiply by the CZ-ladder sign after load
+  F_CZ_POST     = 1u << 11,  // multiply by the CZ-ladder sign before store
+  F_PHASE_PRE   = 1u << 12,  // A *= exp(-i*phase_gamma*D(x)) after load (QAOA cost phase)
+  F_PHASE_POST  = 1u << 13,  // A,B *= exp(-i*phase_gamma*D(x)) before store (inverse phase)
+  F_DRED_POST   = 1u << 14,  // extra += 2*sum D(x) Im(conj(B)A) before the post phase (dE/dgamma)
+  F_OBS_DIAG    = 1u << 15,  // B = D(x)*A at the end; extra += D(x)|A|^2 (lambda = O psi, E)
+  F_PHASE_LUT   = 1u << 16,  // D(x) is integer valued: phase from lut[D + lut_off]
+};
+
+// One term of the diagonal quadratic form D(x) = sum w * z_a(x) [* z_b(x)],
+// with z_p(x) = +1 if global bit p of x is 0 else -1.  b = -1 for a one-bit term.
+struct DTerm {
+  int32_t a, b;
+  double w;
+};
+
+struct PassDesc {
+  // geometry: local bits = [0, c) U [p, p + K - c); tile bits = the rest, in order
+  int32_t n, K, t, c, p;
+  uint32_t flags;
+  int32_t grid;          // stride of the partial rows (>= number of workgroups)
+  int32_t row_base;      // first partial row written by this pass (-1: none)
+  const void* a_in;
+  void* a_out;
+  const void* b_in;
+  void* b_out;
+  // gates, indexed by local bit: U = [[c, ...]] with the sign of s chosen by the planner
+  double gc[kMaxK];
+  double gs[kMaxK];
+  int32_t gtype[kMaxK];
+  int32_t own[kMaxK];    // 1: accumulate the reduction of this local bit into row_base + bit
+  const uint32_t* map;   // ladder map columns (n entries), or null
+  const double* init_tab;// n + 1 entries when F_INIT_TAB
+  const DTerm* dterms;   // diagonal quadratic form
+  int32_t ndterms;
+  int32_t lut_off;       // LUT index offset (F_PHASE_LUT)
+  double phase_gamma;    // angle multiplying D(x) in the phase
+  const double* lut;     // (re, im) pairs: exp(-i*gamma*d) at d = idx - lut_off
+  double* part;          // partial sums: part[row * grid + block]
+};
+
+}  // namespace qg

@@ -1,0 +1,256 @@
+// Elementwise HIP kernels behind the device-resident State API and the
+// generic (non-diagonal) observable path.  One thread per amplitude or pair;
+// these are not the grad_run hot loop (qg_kernels.hip is), but they keep every
+// State method of the reference on the GPU.
+#include <hip/hip_runtime.h>
+#include "qg_internal.h"
+
+namespace qg {
+
+template <class S>
+struct V2;
+template <>
+struct V2<float> {
+  typedef float2 T;
+};
+template <>
+struct V2<double> {
+  typedef double2 T;
+};
+
+// x0' = c x0 - i s (P x)_0 ; x1' = c x1 - i s (P x)_1 on global bit b
+// (every 2x2 of the reference's rotations and their derivatives has this form).
+template <class S>
+__global__ void k_rot1(typename V2<S>::T* v, uint32_t npairs, int b, int type, double c, double s) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npairs) return;
+  const uint32_t lo = i & ((1u << b) - 1u);
+  const uint32_t i0 = ((i >> b) << (b + 1)) | lo;
+  const uint32_t i1 = i0 | (1u << b);
+  const double ar = v[i0].x, ai = v[i0].y, br = v[i1].x, bi = v[i1].y;
+  double r0, j0, r1, j1;
+  if (type == G_X) {
+    r0 = c * ar + s * bi; j0 = c * ai - s * br;
+    r1 = c * br + s * ai; j1 = c * bi - s * ar;
+  } else if (type == G_Y) {
+    r0 = c * ar - s * br; j0 = c * ai - s * bi;
+    r1 = c * br + s * ar; j1 = c * bi + s * ai;
+  } else {
+    r0 = c * ar + s * ai; j0 = c * ai - s * ar;
+    r1 = c * br - s * bi; j1 = c * bi + s * br;
+  }
+  v[i0].x = (S)r0; v[i0].y = (S)j0;
+  v[i1].x = (S)r1; v[i1].y = (S)j1;
+}
+
+// out[y] = in[map(y)] for a GF(2)-linear index map given by its columns.
+template <class S>
+__global__ void k_permute(const typename V2<S>::T* __restrict__ in, typename V2<S>::T* __restrict__ out,
+                          uint32_t N, const uint32_t* __restrict__ map, int n) {
+  const uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
+  if (y >= N) return;
+  uint32_t x = 0;
+  for (int k = 0; k < n; ++k) x ^= map[k] & (0u - ((y >> k) & 1u));
+  out[y] = in[x];
+}
+
+__device__ __forceinline__ double diag_at(const DTerm* __restrict__ t, int nt, uint32_t x) {
+  double D = 0.0;
+  for (int k = 0; k < nt; ++k) {
+    double z = ((x >> t[k].a) & 1u) ? -1.0 : 1.0;
+    if (t[k].b >= 0) z *= ((x >> t[k].b) & 1u) ? -1.0 : 1.0;
+    D += t[k].w * z;
+  }
+  return D;
+}
+
+// v *= exp(-i * angle * D(x))   (and CZ-ladder sign when cz != 0 and nt == 0)
+template <class S>
+__global__ void k_diag_phase(typename V2<S>::T* v, uint32_t N, const DTerm* __restrict__ t, int nt, double angle,
+                             int cz, int n) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= N) return;
+  double re = v[x].x, im = v[x].y;
+  if (cz) {
+    const uint32_t chain = (n >= 2) ? (uint32_t)((1ull << (n - 1)) - 1ull) : 0u;
+    if (__popc(x & (x >> 1) & chain) & 1) { re = -re; im = -im; }
+  } else {
+    double sn, cs;
+    sincos(angle * diag_at(t, nt, x), &sn, &cs);
+    const double r = cs * re + sn * im, i = cs * im - sn * re;
+    re = r; im = i;
+  }
+  v[x].x = (S)re;
+  v[x].y = (S)im;
+}
+
+// out = O in, O = D(x) + sum_k w_k P_k (P in {X, Y} on global bit b_k).
+template <class S>
+__global__ void k_obs_apply(const typename V2<S>::T* __restrict__ in, typename V2<S>::T* __restrict__ out,
+                            uint32_t N, const DTerm* __restrict__ dt, int ndt, const DTerm* __restrict__ xy, int nxy) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= N) return;
+  const double D = diag_at(dt, ndt, x);
+  double re = D * (double)in[x].x, im = D * (double)in[x].y;
+  for (int k = 0; k < nxy; ++k) {
+    const int b = xy[k].a;
+    const double w = xy[k].w;
+    const uint32_t y = x ^ (1u << b);
+    const double pr = in[y].x, pi = in[y].y;
+    if (xy[k].b == 0) {  // X
+      re += w * pr; im += w * pi;
+    } else {             // Y: (Y v)_x = -i v_y if bit 0, +i v_y if bit 1
+      const double sg = ((x >> b) & 1u) ? 1.0 : -1.0;
+      re += w * (-sg * pi); im += w * (sg * pr);
+    }
+  }
+  out[x].x = (S)re;
+  out[x].y = (S)im;
+}
+
+// part[block] = sum over this block's elements of Re(conj(a) b) (mode 0),
+// |a|^2 (mode 1), in FP64, fixed-order tree.
+template <class S>
+__global__ void k_dot(const typename V2<S>::T* __restrict__ a, const typename V2<S>::T* __restrict__ b, uint32_t N,
+                      int mode, double* __restrict__ part) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+    const double ar = a[i].x, ai = a[i].y;
+    if (mode == 0) acc += ar * (double)b[i].x + ai * (double)b[i].y;
+    else acc += ar * ar + ai * ai;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void k_sum(const double* __restrict__ part, int np, double* __restrict__ out) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < np; ++i) s += part[i];
+    *out = s;
+  }
+}
+
+// amp(x) = tab[popcount(x)]  (product initial states)
+template <class S>
+__global__ void k_init(typename V2<S>::T* v, uint32_t N, const double* __restrict__ tab) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= N) return;
+  v[x].x = (S)tab[__popc(x)];
+  v[x].y = (S)0.0;
+}
+
+template <class S>
+__global__ void k_convert_in(const double2* __restrict__ in, typename V2<S>::T* __restrict__ out, uint32_t N) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= N) return;
+  out[x].x = (S)in[x].x;
+  out[x].y = (S)in[x].y;
+}
+
+template <class S>
+__global__ void k_convert_out(const typename V2<S>::T* __restrict__ in, double2* __restrict__ out, uint32_t N) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= N) return;
+  out[x].x = (double)in[x].x;
+  out[x].y = (double)in[x].y;
+}
+
+template <class S>
+__global__ void k_scale(typename V2<S>::T* v, uint32_t N, double re, double im) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= N) return;
+  const double a = v[x].x, b = v[x].y;
+  v[x].x = (S)(re * a - im * b);
+  v[x].y = (S)(re * b + im * a);
+}
+
+// ---------------------------------------------------------------------------
+static inline dim3 grid_for(uint64_t n, int threads) { return dim3((unsigned)((n + threads - 1) / threads)); }
+
+#define QG_DT_SWITCH(dtype, CALL) \
+  do {                            \
+    if ((dtype) == 1) {           \
+      typedef float S;            \
+      CALL;                       \
+    } else {                      \
+      typedef double S;           \
+      CALL;                       \
+    }                             \
+  } while (0)
+
+hipError_t simple_rot1(int dtype, void* v, int n, int b, int type, double c, double s, hipStream_t st) {
+  const uint32_t np = (uint32_t)(1ull << (n - 1));
+  QG_DT_SWITCH(dtype, hipLaunchKernelGGL(k_rot1<S>, grid_for(np, 256), dim3(256), 0, st,
+                                         (typename V2<S>::T*)v, np, b, type, c, s));
+  return hipGetLastError();
+}
+
+hipError_t simple_permute(int dtype, const void* in, void* out, int n, const uint32_t* dmap, hipStream_t st) {
+  const uint32_t N = (uint32_t)(1ull << n);
+  QG_DT_SWITCH(dtype, hipLaunchKernelGGL(k_permute<S>, grid_for(N, 256), dim3(256), 0, st,
+                                         (const typename V2<S>::T*)in, (typename V2<S>::T*)out, N, dmap, n));
+  return hipGetLastError();
+}
+
+hipError_t simple_diag_phase(int dtype, void* v, int n, const DTerm* dt, int nt, double angle, int cz,
+                             hipStream_t st) {
+  const uint32_t N = (uint32_t)(1ull << n);
+  QG_DT_SWITCH(dtype, hipLaunchKernelGGL(k_diag_phase<S>, grid_for(N, 256), dim3(256), 0, st,
+                                         (typename V2<S>::T*)v, N, dt, nt, angle, cz, n));
+  return hipGetLastError();
+}
+
+hipError_t simple_obs_apply(int dtype, const void* in, void* out, int n, const DTerm* dt, int ndt, const DTerm* xy,
+                            int nxy, hipStream_t st) {
+  const uint32_t N = (uint32_t)(1ull << n);
+  QG_DT_SWITCH(dtype, hipLaunchKernelGGL(k_obs_apply<S>, grid_for(N, 256), dim3(256), 0, st,
+                                         (const typename V2<S>::T*)in, (typename V2<S>::T*)out, N, dt, ndt, xy, nxy));
+  return hipGetLastError();
+}
+
+// out = sum: mode 0 Re<a|b>, mode 1 <a|a>; part needs `kDotBlocks` doubles.
+constexpr int kDotBlocks = 512;
+hipError_t simple_dot(int dtype, const void* a, const void* b, int n, int mode, double* part, double* out,
+                      hipStream_t st) {
+  const uint32_t N = (uint32_t)(1ull << n);
+  const int blocks = (int)((N + 255) / 256) < kDotBlocks ? (int)((N + 255) / 256) : kDotBlocks;
+  QG_DT_SWITCH(dtype, hipLaunchKernelGGL(k_dot<S>, dim3(blocks), dim3(256), 0, st, (const typename V2<S>::T*)a,
+                                         (const typename V2<S>::T*)b, N, mode, part));
+  hipLaunchKernelGGL(k_sum, dim3(1), dim3(1), 0, st, (const double*)part, blocks, out);
+  return hipGetLastError();
+}
+
+hipError_t simple_init(int dtype, void* v, int n, const double* tab, hipStream_t st) {
+  const uint32_t N = (uint32_t)(1ull << n);
+  QG_DT_SWITCH(dtype, hipLaunchKernelGGL(k_init<S>, grid_for(N, 256), dim3(256), 0, st, (typename V2<S>::T*)v, N, tab));
+  return hipGetLastError();
+}
+
+hipError_t simple_scale(int dtype, void* v, int n, double re, double im, hipStream_t st) {
+  const uint32_t N = (uint32_t)(1ull << n);
+  QG_DT_SWITCH(dtype, hipLaunchKernelGGL(k_scale<S>, grid_for(N, 256), dim3(256), 0, st, (typename V2<S>::T*)v, N, re, im));
+  return hipGetLastError();
+}
+
+hipError_t simple_convert_in(int dtype, const void* in_c128, void* out, int n, hipStream_t st) {
+  const uint32_t N = (uint32_t)(1ull << n);
+  QG_DT_SWITCH(dtype, hipLaunchKernelGGL(k_convert_in<S>, grid_for(N, 256), dim3(256), 0, st, (const double2*)in_c128,
+                                         (typename V2<S>::T*)out, N));
+  return hipGetLastError();
+}
+
+hipError_t simple_convert_out(int dtype, const void* in, void* out_c128, int n, hipStream_t st) {
+  const uint32_t N = (uint32_t)(1ull << n);
+  QG_DT_SWITCH(dtype, hipLaunchKernelGGL(k_convert_out<S>, grid_for(N, 256), dim3(256), 0, st,
+                                         (const typename V2<S>::T*)in, (double2*)out_c128, N));
+  return hipGetLastError();
+}
+
+}  // namespace qg

@@ -1,0 +1,110 @@
+"""GPU parity of McClean.grad_run / run_expec_val against the CPU oracle.
+
+Bar: complex128 (the plain method) within 1e-12 norm-relative of the oracle's
+complex128 gradient and |dE| <= 1e-12 * max(|E|, sum|w|); complex64 (FP64
+arithmetic, complex64 state) within 1e-6 norm-relative.
+"""
+import numpy as np
+import pytest
+
+from oracle import statevector as sv
+from tests.conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+C128_TOL = 1e-12
+C64_TOL = 1e-6
+
+
+def _case(n, L, seed, obs_kind='chain'):
+    rs = np.random.RandomState(seed)
+    axes, angles = sv.mcclean_random_params(L, n, rs)
+    if obs_kind == 'chain':
+        obs = sv.zz_chain_observable(n, rs.standard_normal(max(n - 1, 0))) if n > 1 else {'z': np.array([0.7])}
+    elif obs_kind == 'z0z1':
+        zz = np.full((n, n), None)
+        zz[0, 1] = 1.0
+        obs = {'zz': zz}
+    elif obs_kind == 'zmix':
+        obs = sv.zz_chain_observable(n, rs.standard_normal(n - 1))
+        obs['z'] = np.array([rs.standard_normal() if q % 2 == 0 else None for q in range(n)], dtype=object)
+    return axes, angles, obs
+
+
+def _run(n, L, axes, angles, obs, dtype='complex128', entangler='cnot', ini_state=None):
+    from qradient_amd.circuit_logic import McClean
+    c = McClean(n, obs, L, axes=axes, angles=angles, dtype=dtype, entangler=entangler)
+    return c.grad_run(ini_state=ini_state)
+
+
+@pytest.mark.parametrize('n,L,seed', [(1, 3, 0), (2, 2, 1), (3, 4, 2), (4, 3, 3), (5, 5, 4), (8, 6, 5),
+                                      (10, 10, 6), (12, 5, 7), (13, 4, 8), (14, 4, 9), (16, 3, 10),
+                                      (17, 2, 11), (20, 2, 12)])
+@pytest.mark.parametrize('entangler', ['cnot', 'cz'])
+def test_grad_run_c128_matches_oracle(gpu, n, L, seed, entangler):
+    axes, angles, obs = _case(n, L, seed)
+    E, g = _run(n, L, axes, angles, obs, entangler=entangler)
+    Eo, go = sv.mcclean_grad_run_adjoint(n, axes, angles, obs, entangler=entangler)
+    wsum = sum(abs(float(w)) for w in obs.get('zz', np.array([])).ravel() if w is not None) + 1.0
+    assert abs(E - Eo) <= C128_TOL * max(abs(Eo), wsum), (E, Eo)
+    assert g.shape == (L, n)
+    assert rel_err(g, go) <= C128_TOL, rel_err(g, go)
+
+
+@pytest.mark.parametrize('n,L,seed', [(6, 4, 20), (10, 10, 21), (14, 6, 22), (18, 4, 23)])
+def test_grad_run_c64_within_tolerance(gpu, n, L, seed):
+    axes, angles, obs = _case(n, L, seed)
+    E, g = _run(n, L, axes, angles, obs, dtype='complex64')
+    Eo, go = sv.mcclean_grad_run_adjoint(n, axes, angles, obs)
+    assert abs(E - Eo) <= C64_TOL * max(abs(Eo), 1.0)
+    assert rel_err(g, go) <= C64_TOL, rel_err(g, go)
+
+
+def test_config1_z0z1(gpu):
+    """McClean n=10, L=10, observable Z0Z1, complex128: E and all 100 gradients."""
+    axes, angles, obs = _case(10, 10, 1234, 'z0z1')
+    E, g = _run(10, 10, axes, angles, obs)
+    Eo, go = sv.mcclean_grad_run(10, axes, angles, obs)
+    assert abs(E - Eo) <= C128_TOL
+    assert rel_err(g, go) <= C128_TOL
+
+
+def test_z_terms_and_ini_state(gpu):
+    n, L = 7, 3
+    axes, angles, obs = _case(n, L, 31, 'zmix')
+    rs = np.random.RandomState(5)
+    ini = rs.standard_normal(2 ** n) + 1j * rs.standard_normal(2 ** n)
+    ini /= np.linalg.norm(ini)
+    keep = ini.copy()
+    E, g = _run(n, L, axes, angles, obs, ini_state=ini)
+    Eo, go = sv.mcclean_grad_run(n, axes, angles, obs, ini_state=ini)
+    assert np.array_equal(ini, keep)
+    assert abs(E - Eo) <= 1e-12 * max(1.0, abs(Eo))
+    assert rel_err(g, go) <= C128_TOL
+
+
+def test_run_expec_val(gpu):
+    from qradient_amd.circuit_logic import McClean
+    n, L = 9, 4
+    axes, angles, obs = _case(n, L, 41)
+    c = McClean(n, obs, L, axes=axes, angles=angles)
+    assert abs(c.run_expec_val() - sv.mcclean_run_expec_val(n, axes, angles, obs)) <= 1e-12
+
+
+def test_repeated_calls_are_bitwise_identical(gpu):
+    n, L = 15, 3
+    axes, angles, obs = _case(n, L, 51)
+    from qradient_amd.circuit_logic import McClean
+    c = McClean(n, obs, L, axes=axes, angles=angles)
+    E1, g1 = c.grad_run()
+    E2, g2 = c.grad_run()
+    assert E1 == E2 and np.array_equal(g1, g2)
+
+
+def test_invalid_axis_raises(gpu):
+    from qradient_amd.circuit_logic import McClean
+    axes, angles, obs = _case(4, 2, 3)
+    axes[1, 2] = 3
+    c = McClean(4, obs, 2, axes=axes, angles=angles)
+    with pytest.raises(ValueError):
+        c.grad_run()

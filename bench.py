@@ -1,0 +1,173 @@
+"""Benchmark: McClean grad_run, 24 qubits, 40 layers (960 parameters).
+
+Workload: seeded random Pauli axes and uniform angles in [0, 2 pi), observable
+sum_i w_i Z_i Z_{i+1} with w ~ N(0, 1), CNOT-ladder entangler.  One "step" is
+one full grad_run (forward sweep, lambda = O psi, adjoint sweep; E plus all 960
+gradients), inputs resident on the device.  ``value`` is the complex64-state
+(FP64 arithmetic and reductions) throughput in circuits/s; the plain
+complex128 method is timed on the same circuit and reported beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints one JSON line.  ``roofline`` is measured live: HIP events around every
+pass launch of the timed steps (on the library's stream) give the average
+duration of the dominant kernel; its algorithmic bytes per launch come from
+the pass's state reads and writes.  ``cpu_baseline`` times the C restatement
+of the reference algorithm (oracle/c/oracle_ref.c) on one core over a bounded
+sample (the first layer of the same circuit) and scales it to the full
+circuit.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_QUBITS, N_LAYERS, SEED = 24, 40, 2024
+HBM_PEAK_GBS = 8000.0
+
+
+def workload():
+    rs = np.random.RandomState(SEED)
+    axes = (3 * rs.rand(N_LAYERS, N_QUBITS)).astype(int)
+    angles = 2 * np.pi * rs.rand(N_LAYERS, N_QUBITS)
+    w = rs.standard_normal(N_QUBITS - 1)
+    zz = np.full((N_QUBITS, N_QUBITS), None)
+    for i in range(N_QUBITS - 1):
+        zz[i, i + 1] = float(w[i])
+    return axes, angles, {'zz': zz}
+
+
+def pass_times(circ, native):
+    cnt = ctypes.c_int()
+    native.call('qg_plan_pass_times', circ._plan, 0, None, None, ctypes.byref(cnt))
+    ms = (ctypes.c_float * cnt.value)()
+    kinds = (ctypes.c_int * cnt.value)()
+    native.call('qg_plan_pass_times', circ._plan, cnt.value, ms, kinds, ctypes.byref(cnt))
+    return np.array(ms[:]), np.array(kinds[:])
+
+
+def time_steps(circ, steps, warmup, sync, profile):
+    from qradient_amd import _native
+    for _ in range(warmup):
+        circ.grad_run()
+    if profile:
+        _native.call('qg_plan_set_profiling', circ._plan, 1)
+    per = []
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        E, g = circ.grad_run()
+        if profile:
+            per.append(pass_times(circ, _native))
+    sync()
+    dt = time.perf_counter() - t0
+    if profile:
+        _native.call('qg_plan_set_profiling', circ._plan, 0)
+    return dt, E, g, per
+
+
+def roofline(per, state_bytes):
+    """Dominant pass class: reverse two-state passes (read A, read B, write B)."""
+    ms = np.concatenate([p[0] for p in per])
+    kinds = np.concatenate([p[1] for p in per])
+    alg = {0: 2 * state_bytes, 1: 3 * state_bytes, 2: 3 * state_bytes}
+    totals = {k: ms[kinds == k].sum() for k in set(kinds.tolist())}
+    dom = max(totals, key=totals.get)
+    avg_ms = float(ms[kinds == dom].mean())
+    achieved = alg[dom] / (avg_ms * 1e-3) / 1e9
+    return dom, avg_ms, achieved, alg[dom], {int(k): float(v) / len(per) for k, v in totals.items()}
+
+
+def cpu_baseline(axes, angles, obs):
+    from threadpoolctl import threadpool_limits
+    from oracle import c_oracle
+    layers = 1
+    with threadpool_limits(1):
+        t0 = time.perf_counter()
+        c_oracle.mcclean_grad_run(N_QUBITS, axes, angles, obs, layers=layers)
+        dt = time.perf_counter() - t0
+    per_circuit = dt * N_LAYERS / layers
+    return {'value': 1.0 / per_circuit, 'unit': 'circuits/s', 'cores': 1, 'kind': 'port',
+            'sample': 'first {} of {} layers of the same 24-qubit circuit (forward + adjoint sweep of that '
+                      'layer, reference algorithm with stored history), C restatement oracle/c/oracle_ref.c, '
+                      '1 thread; {:.1f} s measured, scaled x{} to the full circuit'.format(layers, N_LAYERS, dt,
+                                                                                        N_LAYERS // layers)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=10)
+    ap.add_argument('--warmup', type=int, default=2)
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py measures one GPU: this build has no multi-GPU path (every configuration is a '
+              'single-GPU workload); run with --gpus 1', file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from qradient_amd.circuit_logic import McClean
+
+    def sync():
+        torch.cuda.synchronize()
+
+    axes, angles, obs = workload()
+    res = {}
+    grads = {}
+    for dtype in ('complex64', 'complex128'):
+        circ = McClean(N_QUBITS, obs, N_LAYERS, axes=axes, angles=angles, dtype=dtype)
+        steps = args.steps if dtype == 'complex64' else max(2, args.steps // 2)
+        dt, E, g, per = time_steps(circ, steps, args.warmup, sync, profile=True)
+        sb = (2 ** N_QUBITS) * (8 if dtype == 'complex64' else 16)
+        res[dtype] = dict(ms=dt / steps * 1e3, E=E, roof=roofline(per, sb), steps=steps)
+        grads[dtype] = g
+        circ._destroy_plan()
+    c64, c128 = res['complex64'], res['complex128']
+    rel = float(np.linalg.norm(grads['complex64'] - grads['complex128']) / np.linalg.norm(grads['complex128']))
+    dom, avg_ms, achieved, alg_bytes, totals = c64['roof']
+    traffic = None
+    tfile = os.path.join(ROOT, 'profiles', 'traffic_mcclean_n24_L40.json')
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get('c64_reverse_pass_bytes')
+    out = {
+        'metric': 'grad_run circuits/s (McClean 24 qubits, 40 layers, complex64 state + FP64 reductions)',
+        'value': 1e3 / c64['ms'],
+        'unit': 'circuits/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': c64['ms'],
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'c64 (FP64 arithmetic and reductions)',
+        'data': 'synthetic (seeded random McClean circuit and N(0,1) ZZ-chain weights)',
+        'config': {'workload': 'mcclean_grad_run_n24_L40_zzchain', 'qubits': N_QUBITS, 'layers': N_LAYERS,
+                   'parameters': N_QUBITS * N_LAYERS, 'entangler': 'cnot', 'seed': SEED},
+        'c128_plain': {'circuits_per_s': 1e3 / c128['ms'], 'ms_per_step': c128['ms'], 'steps': c128['steps'],
+                       'roofline_achieved_gbs': c128['roof'][2]},
+        'speedup_c64_vs_c128': c128['ms'] / c64['ms'],
+        'c64_vs_c128_grad_rel_err': rel,
+        'E': c64['E'],
+        'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic,
+                     'kernel': {0: 'forward pass', 1: 'adjoint two-state pass', 2: 'forward + observable pass'}[dom],
+                     'avg_launch_ms': avg_ms, 'algorithmic_bytes_per_launch': alg_bytes,
+                     'ms_per_step_by_kind': totals},
+    }
+    if not args.no_cpu_baseline:
+        out['cpu_baseline'] = cpu_baseline(axes, angles, obs)
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

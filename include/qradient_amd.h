@@ -41,6 +41,8 @@ extern "C" {
 #define QG_ENT_CNOT 0 /* CNOT ladder of state.py:209-251 (odd bonds act first) */
 #define QG_ENT_CZ 1   /* nearest-neighbour CZ ladder (diagonal variant) */
 
+#define QG_HISTORY (-1) /* ckpt_every: keep the per-pass history, fail if it does not fit */
+
 #define QG_AXIS_X 0
 #define QG_AXIS_Y 1
 #define QG_AXIS_Z 2
@@ -77,7 +79,8 @@ int qg_ctx_synchronize(qg_ctx* ctx);
 /* ---- McClean circuit (qradient/circuit_logic/mc_clean.py) ----------------
  * Plan for n qubits and L layers.  ckpt_every = 0 lets the planner keep the
  * per-pass state history when it fits in HBM and otherwise checkpoint; k > 0
- * forces checkpoints every k layers with psi uncomputed in between.
+ * forces checkpoints every k layers with psi uncomputed in between;
+ * QG_HISTORY forces the history (QG_ERR_NOMEM if it does not fit).
  * Replaces the construction of McClean.__init__ (mc_clean.py:9-25). */
 int qg_mcclean_plan(qg_ctx* ctx, int n, int L, int dtype, int entangler, int ckpt_every, qg_plan** out);
 
@@ -115,6 +118,17 @@ int qg_plan_set_workspace(qg_plan* plan, void* dev_ptr, size_t bytes);
 int qg_plan_get_state(qg_plan* plan, double* out);
 /* Device time of the last run in milliseconds (hipEvents around the launches). */
 int qg_plan_last_ms(const qg_plan* plan, double* ms);
+/* Record a HIP event before every kernel launch of later runs (on = 1). */
+int qg_plan_set_profiling(qg_plan* plan, int on);
+/* Per-launch device times (ms) of the last profiled run and their kind
+ * (0 forward pass, 1 reverse two-state pass, 2 forward pass fused with the
+ * observable); *count receives the launch count, at most cap are written. */
+int qg_plan_pass_times(const qg_plan* plan, int cap, float* ms, int* kinds, int* count);
+/* Tile shape of a plan: kmax local bits per pass, at least cmin contiguous
+ * low bits in high-bit passes, and log2 amplitudes per thread (4 or 5) of the
+ * complex64 forward and history-adjoint passes.  Used by the autotuner;
+ * rejects shapes that change the number of passes per layer. */
+int qg_plan_tune(qg_plan* plan, int kmax, int cmin, int r_fwd, int r_rev);
 int qg_plan_destroy(qg_plan* plan);
 
 /* ---- device-resident State (qradient/physical_components/state.py) --------

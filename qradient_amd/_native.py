@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'libqradient
 QG_OK, QG_ERR_INVALID, QG_ERR_UNSUPPORTED, QG_ERR_HIP, QG_ERR_NOMEM = 0, 1, 2, 3, 4
 QG_C128, QG_C64 = 0, 1
 QG_ENT_CNOT, QG_ENT_CZ = 0, 1
+QG_HISTORY = -1
 TERM_X, TERM_Y, TERM_Z, TERM_ZZ = 0, 1, 2, 3
 
 DTYPES = {'complex128': QG_C128, 'c128': QG_C128, np.complex128: QG_C128,
@@ -52,6 +53,9 @@ SIGNATURES = {
     'qg_plan_set_workspace': [_vp, _vp, ctypes.c_size_t],
     'qg_plan_get_state': [_vp, _dp],
     'qg_plan_last_ms': [_vp, _dp],
+    'qg_plan_set_profiling': [_vp, _i],
+    'qg_plan_pass_times': [_vp, _i, ctypes.POINTER(ctypes.c_float), _ip, _ip],
+    'qg_plan_tune': [_vp, _i, _i, _i, _i],
     'qg_plan_destroy': [_vp],
     'qg_state_create': [_vp, _i, _i, ctypes.c_char, _pp],
     'qg_state_destroy': [_vp],

@@ -1,0 +1,71 @@
+"""Batched grad_run: many QAOA circuits of one shape in one launch sequence.
+
+The reference evaluates one circuit per Python call (qradient/circuit_logic/
+qaoa.py:40-70); here B circuits of the same (n, p) share every kernel launch
+(blockIdx.y selects the circuit), so small states still fill the GPU.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native
+from .physical_components import Observable
+
+
+class QaoaBatch:
+    """Plan for up to ``batch`` QAOA circuits of ``n`` qubits and ``p`` layers."""
+
+    def __init__(self, n, p, batch, dtype='complex64', device=0):
+        self.n, self.p, self.batch, self.dtype = int(n), int(p), int(batch), dtype
+        self._ctx = _native.Context.default(device)
+        h = ctypes.c_void_p()
+        _native.call('qg_qaoa_plan', self._ctx.handle, self.n, self.p, _native.DTYPES[dtype], self.batch,
+                     ctypes.byref(h))
+        self._plan = h
+
+    def __del__(self):
+        h = getattr(self, '_plan', None)
+        if h and _native._lib is not None:
+            _native._lib.qg_plan_destroy(h)
+            self._plan = None
+
+    def _terms(self, observables):
+        terms, off = [], [0]
+        for o in observables:
+            ob = o if isinstance(o, Observable) else Observable(o)
+            if ob.qnum != self.n:
+                raise ValueError('observable has {} qubits, plan has {}'.format(ob.qnum, self.n))
+            terms.extend(ob.classical_terms())
+            off.append(len(terms))
+        arr, _ = _native.terms_array(terms)
+        return arr, np.array(off, dtype=np.int32)
+
+    def _params(self, betas, gammas, B):
+        b = np.ascontiguousarray(betas, dtype=np.float64).reshape(B, self.p)
+        g = np.ascontiguousarray(gammas, dtype=np.float64).reshape(B, self.p)
+        return b, g
+
+    def grad_run(self, observables, betas, gammas):
+        """(E[B], grad[B, p, 2]) with grad[b, i] = (dE/dbeta_i, dE/dgamma_i) of circuit b."""
+        B = len(observables)
+        arr, off = self._terms(observables)
+        b, g = self._params(betas, gammas, B)
+        E = np.empty(B)
+        grad = np.empty((B, self.p, 2))
+        _native.call('qg_qaoa_grad_run', self._plan, B, arr, _native.iptr(off), _native.dptr(b), _native.dptr(g),
+                     _native.dptr(E), _native.dptr(grad))
+        return E, grad
+
+    def expec_val(self, observables, betas, gammas):
+        B = len(observables)
+        arr, off = self._terms(observables)
+        b, g = self._params(betas, gammas, B)
+        E = np.empty(B)
+        _native.call('qg_qaoa_expec_val', self._plan, B, arr, _native.iptr(off), _native.dptr(b), _native.dptr(g),
+                     _native.dptr(E))
+        return E
+
+    def last_device_ms(self):
+        ms = ctypes.c_double()
+        _native.call('qg_plan_last_ms', self._plan, ctypes.byref(ms))
+        return ms.value

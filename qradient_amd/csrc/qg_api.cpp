@@ -15,8 +15,8 @@
 
 namespace qg {
 hipError_t set_pass_attributes();
-hipError_t launch_pass(int dtype, int reg_bits, bool two, int phase, const PassDesc* d, int grid, int threads,
-                       hipStream_t st);
+hipError_t launch_pass(int dtype, int R, bool two, int phase, bool ta_float, const PassDesc* d, int K, int grid,
+                       int batch, int threads, int n, hipStream_t st);
 hipError_t launch_finalize(int nout, const int* ptr, const int* rows, const int* cnt, const double* scale,
                            const double* part, int stride, double* out, hipStream_t st);
 hipError_t simple_rot1(int dtype, void* v, int n, int b, int type, double c, double s, hipStream_t st);
@@ -199,6 +199,25 @@ static std::vector<Geo> layer_geometry(int n, int kmax, int cmin) {
   return g;
 }
 
+// Register bits of a pass by role: complex64 forward and history-adjoint
+// passes hold 32 amplitudes per thread (the adjoint's read-only companion in
+// FP32 registers); passes that propagate two FP64 states hold 16.
+enum Role { ROLE_FWD = 0, ROLE_REV_HIST = 1, ROLE_TWO_FP64 = 2 };
+
+static Geo with_role_r(const Geo& g0, int dtype, int role, int r_fwd, int r_rev, bool* ta_float) {
+  Geo g = g0;
+  int R = 4;
+  if (dtype == QG_C64 && role == ROLE_FWD) R = r_fwd;
+  if (dtype == QG_C64 && role == ROLE_REV_HIST) R = r_rev;
+  g.R = std::min(R, g.K);
+  g.t = g.K - g.R;
+  g.threads = 1 << g.t;
+  // a complex64 history-adjoint pass holds its read-only companion in FP32
+  *ta_float = (dtype == QG_C64 && role == ROLE_REV_HIST && g.R >= 4);
+  return g;
+}
+#define with_role(g0, dtype, role, tf) with_role_r(g0, dtype, role, p->r_fwd, p->r_rev, tf)
+
 static int local_of_global(const Geo& g, int b) {  // -1 if not local
   if (b < g.c) return b;
   if (b >= g.p && b < g.p + (g.K - g.c)) return g.c + (b - g.p);
@@ -226,7 +245,10 @@ static void ladder_map(int n, int stacking, std::vector<uint32_t>& cols) {
 enum PlanKind { PK_MCCLEAN = 0, PK_QAOA = 1 };
 
 struct Launch {
-  int desc;  // index into the descriptor table
+  int desc;  // index into the descriptor table (first of `batch` consecutive ones)
+  int batch = 1;
+  int K = 0;
+  bool ta_float = false;  // FP32 companion registers (complex64 history adjoint pass)
   int R, threads, grid, phase;
   bool two;
   int kind;  // 0 pass kernel, 1 generic observable, 2 convert-in (ini state)
@@ -237,8 +259,13 @@ struct qg_plan {
   int kind = PK_MCCLEAN;
   int n = 0, L = 0, dtype = 0, ent = 0, batch = 1;
   bool history = true;
-  int ckpt = 0;
+  int ckpt = 0;   // checkpoint period in layers (uncompute mode)
+  int ckpt_req = 0;  // requested period (0: history when it fits)
+  int nckpt = 0;
+  size_t off_work = 0, off_ckpt = 0;
   int kmax = kMaxK, cmin = 2;
+  int r_fwd = 5, r_rev = 5;  // amplitudes per thread (log2) of forward / history-adjoint passes (complex64)
+  int chunk = 1;  // circuits resident at once (QAOA batches)
   std::vector<Geo> geo;
   size_t S = 0;  // bytes per state
   // workspace
@@ -257,6 +284,7 @@ struct qg_plan {
   double* d_part = nullptr;
   int gmax = 1, nrows = 0;
   int* d_fin = nullptr;         // ptr | rows | cnt
+  size_t fin_cap = 0;
   double* d_scale = nullptr;
   double* d_out = nullptr;
   double* d_dot = nullptr;      // kDotBlocks + 1
@@ -268,11 +296,17 @@ struct qg_plan {
   int nout = 0;
   std::vector<Launch> launches;
   float last_ms = 0.f;
+  // per-launch profiling: one event before every launch and one after the last
+  bool profile = false;
+  std::vector<hipEvent_t> pev;
+  std::vector<float> pass_ms;  // durations of the last profiled run
+  std::vector<int> pass_kind;  // 0 forward, 1 reverse (two-state), 2 forward+observable
   int last_state = -1;  // 0: final state in hist, used by get_state
   void* last_state_ptr = nullptr;
 };
 
 static void* wsp(qg_plan* p, size_t off) { return p->ws + off; }
+static void free_small(qg_plan* p);
 
 static int ensure_workspace(qg_plan* p) {
   if (p->ws) return QG_OK;
@@ -286,10 +320,14 @@ static int ensure_workspace(qg_plan* p) {
   return QG_OK;
 }
 
+static size_t lds_bytes_of(int dtype, int K, int threads) {
+  return (size_t)(1u << K) * (dtype == QG_C64 ? 8 : 16) + (size_t)((threads + 63) / 64) * (kMaxK + 1) * 8;
+}
+
 static int grid_for_pass(const qg_plan* p, const Geo& g) {
-  const size_t lds = (size_t)(1u << g.K) * sizeof(double2) + (size_t)((g.threads + 63) / 64) * (kMaxK + 1) * 8;
+  const size_t lds = lds_bytes_of(p->dtype, g.K, g.threads);
   int per_cu = (int)std::max<size_t>(1, (160 * 1024) / lds);
-  per_cu = std::min(per_cu, std::max(1, 2048 / g.threads));
+  per_cu = std::min(per_cu, std::max(1, 512 / g.threads));  // 2 waves per SIMD at <= 256 VGPRs
   const uint64_t ntiles = 1ull << (p->n - g.K);
   uint64_t target = (uint64_t)p->ctx->num_cu * per_cu;
   uint64_t grid = 1;
@@ -306,7 +344,8 @@ static int alloc_small(qg_plan* p, int ndesc, int nrows, int nout) {
   HIPCHK(hipMalloc(&p->d_xy, sizeof(DTerm) * p->max_terms));
   HIPCHK(hipMalloc(&p->d_part, sizeof(double) * (size_t)std::max(1, nrows) * p->gmax));
   HIPCHK(hipMemset(p->d_part, 0, sizeof(double) * (size_t)std::max(1, nrows) * p->gmax));
-  HIPCHK(hipMalloc(&p->d_fin, sizeof(int) * (size_t)(3 * (nout + 1) + 3 * nrows + 8)));
+  p->fin_cap = (size_t)(3 * (nout + 1) + 3 * nrows + 8);
+  HIPCHK(hipMalloc(&p->d_fin, sizeof(int) * p->fin_cap));
   HIPCHK(hipMalloc(&p->d_scale, sizeof(double) * std::max(1, nout)));
   HIPCHK(hipMalloc(&p->d_out, sizeof(double) * std::max(1, nout)));
   HIPCHK(hipMalloc(&p->d_dot, sizeof(double) * (kDotBlocks + 1)));
@@ -323,19 +362,8 @@ extern "C" int qg_plan_destroy(qg_plan* p) {
   hipSetDevice(p->ctx->device);
   hipStreamSynchronize(p->ctx->stream);
   if (p->ws_owned && p->ws) hipFree(p->ws);
-  hipFree(p->d_map);
-  hipFree(p->d_init);
-  hipFree(p->d_terms);
-  hipFree(p->d_xy);
-  hipFree(p->d_lut);
-  hipFree(p->d_part);
-  hipFree(p->d_fin);
-  hipFree(p->d_scale);
-  hipFree(p->d_out);
-  hipFree(p->d_dot);
-  hipFree(p->d_desc);
-  if (p->h_desc) hipHostFree(p->h_desc);
-  if (p->h_out) hipHostFree(p->h_out);
+  free_small(p);
+  for (hipEvent_t e : p->pev) hipEventDestroy(e);
   delete p;
   return QG_OK;
 }
@@ -408,10 +436,53 @@ static int run_launches(qg_plan* p, int ndesc) {
   qg_ctx* c = p->ctx;
   HIPCHK(hipMemcpyAsync(p->d_desc, p->h_desc, sizeof(PassDesc) * ndesc, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipEventRecord(c->ev0, c->stream));
-  for (const Launch& l : p->launches) {
-    if (l.kind == 0) {
-      HIPCHK(launch_pass(p->dtype, l.R, l.two, l.phase, p->d_desc + l.desc, l.grid, l.threads, c->stream));
+  const size_t nl = p->launches.size();
+  if (p->profile) {
+    while (p->pev.size() < nl + 1) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      p->pev.push_back(e);
     }
+  }
+  for (size_t k = 0; k < nl; ++k) {
+    const Launch& l = p->launches[k];
+    if (p->profile) HIPCHK(hipEventRecord(p->pev[k], c->stream));
+    if (l.kind == 0) {
+      HIPCHK(launch_pass(p->dtype, l.R, l.two, l.phase, l.ta_float, p->d_desc + l.desc, l.K, l.grid, l.batch,
+                         l.threads, p->n, c->stream));
+    }
+  }
+  if (p->profile) HIPCHK(hipEventRecord(p->pev[nl], c->stream));
+  return QG_OK;
+}
+
+static int collect_profile(qg_plan* p) {
+  if (!p->profile) return QG_OK;
+  const size_t nl = p->launches.size();
+  p->pass_ms.assign(nl, 0.f);
+  p->pass_kind.assign(nl, 0);
+  HIPCHK(hipEventSynchronize(p->pev[nl]));
+  for (size_t k = 0; k < nl; ++k) {
+    HIPCHK(hipEventElapsedTime(&p->pass_ms[k], p->pev[k], p->pev[k + 1]));
+    const PassDesc& d = p->h_desc[p->launches[k].desc];
+    p->pass_kind[k] = (d.flags & F_REDUCE) ? 1 : (d.flags & F_OBS_DIAG) ? 2 : 0;
+  }
+  return QG_OK;
+}
+
+extern "C" int qg_plan_set_profiling(qg_plan* p, int on) {
+  if (!p) return fail(QG_ERR_INVALID, "NULL plan");
+  p->profile = on != 0;
+  return QG_OK;
+}
+
+extern "C" int qg_plan_pass_times(const qg_plan* p, int cap, float* ms, int* kinds, int* count) {
+  if (!p || !count) return fail(QG_ERR_INVALID, "NULL argument");
+  const int n = (int)p->pass_ms.size();
+  *count = n;
+  for (int k = 0; k < n && k < cap; ++k) {
+    if (ms) ms[k] = p->pass_ms[k];
+    if (kinds) kinds[k] = p->pass_kind[k];
   }
   return QG_OK;
 }
@@ -419,6 +490,94 @@ static int run_launches(qg_plan* p, int ndesc) {
 // ---------------------------------------------------------------------------
 // McClean
 // ---------------------------------------------------------------------------
+static void free_small(qg_plan* p) {
+  hipFree(p->d_map);
+  hipFree(p->d_init);
+  hipFree(p->d_terms);
+  hipFree(p->d_xy);
+  hipFree(p->d_lut);
+  hipFree(p->d_part);
+  hipFree(p->d_fin);
+  hipFree(p->d_scale);
+  hipFree(p->d_out);
+  hipFree(p->d_dot);
+  hipFree(p->d_desc);
+  if (p->h_desc) hipHostFree(p->h_desc);
+  if (p->h_out) hipHostFree(p->h_out);
+  p->d_map = nullptr;
+  p->d_init = nullptr;
+  p->d_terms = p->d_xy = nullptr;
+  p->d_lut = p->d_part = p->d_scale = p->d_out = p->d_dot = nullptr;
+  p->d_fin = nullptr;
+  p->d_desc = nullptr;
+  p->h_desc = nullptr;
+  p->h_out = nullptr;
+  p->lut_len = 0;
+  p->fin_cap = 0;
+}
+
+// Geometry, memory mode (history or checkpointed uncompute), workspace layout
+// and the small device tables of a McClean plan.
+static int mcclean_layout(qg_plan* p) {
+  const int n = p->n, L = p->L;
+  p->geo = layer_geometry(n, p->kmax, p->cmin);
+  const int P = (int)p->geo.size();
+  p->S = (size_t)(1ull << n) * (p->dtype == QG_C64 ? 8 : 16);
+  size_t free_b = 0, total_b = 0;
+  hipMemGetInfo(&free_b, &total_b);
+  if (p->ws_owned && p->ws) free_b += p->ws_bytes;
+  const size_t budget = (size_t)(0.85 * (double)free_b);
+  const size_t hist_bytes = (size_t)(P * L + 4) * p->S;
+  if (p->ckpt_req == QG_HISTORY && hist_bytes > budget)
+    return fail(QG_ERR_NOMEM, "the per-pass history does not fit in device memory");
+  if (p->ckpt_req == QG_HISTORY || (p->ckpt_req == 0 && hist_bytes <= budget)) {
+    // every pass output is kept: the adjoint sweep reads psi, never recomputes it
+    p->history = true;
+    p->off_hist = 0;
+    p->off_lam = (size_t)P * L * p->S;
+  } else {
+    // psi is uncomputed in the adjoint sweep and restarted from a stored
+    // checkpoint at the end of every k-th layer (and of the last layer)
+    int k = p->ckpt_req;
+    if (k == 0) {
+      for (k = 1; k <= L; ++k)
+        if ((size_t)(6 + (L + k - 1) / k) * p->S <= budget) break;
+      if (k > L) k = L;
+    }
+    p->history = false;
+    p->ckpt = std::min(k, L);
+    p->nckpt = 0;
+    for (int i = 0; i < L; ++i)
+      if ((i + 1) % p->ckpt == 0 || i == L - 1) p->nckpt++;
+    if ((size_t)(6 + p->nckpt) * p->S > budget)
+      return fail(QG_ERR_NOMEM, "circuit does not fit in device memory even with one checkpoint");
+    p->off_work = 0;
+    p->off_ckpt = 2 * p->S;
+    p->off_lam = p->off_ckpt + (size_t)p->nckpt * p->S;
+  }
+  p->off_ini = p->off_lam + 2 * p->S;
+  p->off_prep = p->off_ini + p->S;
+  p->ws_bytes = p->off_prep + p->S;
+  p->gmax = 1;
+  for (const Geo& g : p->geo)
+    for (int role = 0; role < 3; ++role) {
+      bool tf;
+      p->gmax = std::max(p->gmax, grid_for_pass(p, with_role(g, p->dtype, role, &tf)));
+    }
+  const int npass = 2 * P * L + 2 * P;  // forward + reverse + ini-state preparation
+  p->nrows = (P * L + 1) * (kMaxK + 1);
+  CHK(alloc_small(p, npass, p->nrows, 1 + L * n));
+  // static tables
+  std::vector<uint32_t> map;
+  ladder_map(n, 1, map);
+  std::vector<double> init(n + 1);
+  const double c8 = std::cos(M_PI / 8), s8 = std::sin(M_PI / 8);
+  for (int k = 0; k <= n; ++k) init[k] = std::pow(c8, n - k) * std::pow(s8, k);
+  HIPCHK(hipMemcpy(p->d_map, map.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p->d_init, init.data(), sizeof(double) * (n + 1), hipMemcpyHostToDevice));
+  return QG_OK;
+}
+
 extern "C" int qg_mcclean_plan(qg_ctx* ctx, int n, int L, int dtype, int entangler, int ckpt_every,
                                qg_plan** out) {
   if (!ctx || !out) return fail(QG_ERR_INVALID, "NULL argument");
@@ -426,7 +585,7 @@ extern "C" int qg_mcclean_plan(qg_ctx* ctx, int n, int L, int dtype, int entangl
   if (L < 1) return fail(QG_ERR_INVALID, "layer number must be >= 1");
   if (dtype != QG_C128 && dtype != QG_C64) return fail(QG_ERR_INVALID, "dtype must be QG_C128 or QG_C64");
   if (entangler != QG_ENT_CNOT && entangler != QG_ENT_CZ) return fail(QG_ERR_INVALID, "unknown entangler");
-  if (ckpt_every < 0) return fail(QG_ERR_INVALID, "ckpt_every must be >= 0");
+  if (ckpt_every < QG_HISTORY) return fail(QG_ERR_INVALID, "ckpt_every must be >= 0 or QG_HISTORY");
   HIPCHK(hipSetDevice(ctx->device));
   qg_plan* p = new qg_plan();
   p->ctx = ctx;
@@ -435,41 +594,11 @@ extern "C" int qg_mcclean_plan(qg_ctx* ctx, int n, int L, int dtype, int entangl
   p->L = L;
   p->dtype = dtype;
   p->ent = entangler;
-  p->geo = layer_geometry(n, p->kmax, p->cmin);
-  const int P = (int)p->geo.size();
-  p->S = (size_t)(1ull << n) * (dtype == QG_C64 ? 8 : 16);
-  size_t free_b = 0, total_b = 0;
-  hipMemGetInfo(&free_b, &total_b);
-  const size_t hist_bytes = (size_t)(P * L + 4) * p->S;
-  if (ckpt_every > 0 || hist_bytes > (size_t)(0.85 * (double)free_b)) {
-    delete p;
-    return fail(QG_ERR_UNSUPPORTED, "checkpointed (uncompute) mode is not available in this build for this size");
-  }
-  p->history = true;
-  p->off_hist = 0;
-  p->off_lam = (size_t)P * L * p->S;
-  p->off_ini = p->off_lam + 2 * p->S;
-  p->off_prep = p->off_ini + p->S;
-  p->ws_bytes = p->off_prep + p->S;
-  for (const Geo& g : p->geo) p->gmax = std::max(p->gmax, grid_for_pass(p, g));
-  const int npass = 2 * P * L + 2 * P;  // forward + reverse + ini-state preparation
-  p->nrows = (P * L + 1) * (kMaxK + 1);
-  int r = alloc_small(p, npass, p->nrows, 1 + L * n);
+  p->ckpt_req = ckpt_every;
+  int r = mcclean_layout(p);
   if (r != QG_OK) {
     qg_plan_destroy(p);
     return r;
-  }
-  // static tables
-  std::vector<uint32_t> map;
-  ladder_map(n, 1, map);
-  std::vector<double> init(n + 1);
-  const double c8 = std::cos(M_PI / 8), s8 = std::sin(M_PI / 8);
-  for (int k = 0; k <= n; ++k) init[k] = std::pow(c8, n - k) * std::pow(s8, k);
-  hipError_t e = hipMemcpy(p->d_map, map.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(p->d_init, init.data(), sizeof(double) * (n + 1), hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    qg_plan_destroy(p);
-    return fail(QG_ERR_HIP, std::string("plan tables: ") + hipGetErrorString(e));
   }
   *out = p;
   return QG_OK;
@@ -484,17 +613,28 @@ static int mcclean_schedule(qg_plan* p, const int32_t* axes, const double* angle
   p->launches.clear();
   int nd = 0;
   int row = 0;
-  std::vector<int> fin_rows(1 + L * n, -1);
-  std::vector<int> row_cnt(p->nrows, 0);
   auto hist = [&](int i, int g) { return wsp(p, p->off_hist + ((size_t)i * P + g) * p->S); };
   void* lam[2] = {wsp(p, p->off_lam), wsp(p, p->off_lam + p->S)};
+  void* work[2] = {wsp(p, p->off_work), wsp(p, p->off_work + p->S)};
+  std::vector<int> slot(L, -1);
+  if (!p->history) {
+    int m = 0;
+    for (int i = 0; i < L; ++i)
+      if ((i + 1) % p->ckpt == 0 || i == L - 1) slot[i] = m++;
+  }
+  auto ckpt = [&](int i) { return wsp(p, p->off_ckpt + (size_t)slot[i] * p->S); };
+  auto other = [&](const void* b) { return b == work[0] ? work[1] : work[0]; };
   const uint32_t ent_flag_fwd = p->ent == QG_ENT_CNOT ? F_GATHER : F_CZ_PRE;
   const uint32_t ent_flag_rev = p->ent == QG_ENT_CNOT ? F_SCATTER : F_CZ_POST;
 
-  auto push = [&](const Geo& g, bool two) -> PassDesc& {
+  auto push = [&](const Geo& g0, bool two, int role) -> PassDesc& {
+    bool tf = false;
+    const Geo g = with_role(g0, p->dtype, role, &tf);
     Launch l;
     l.desc = nd;
     l.R = g.R;
+    l.K = g.K;
+    l.ta_float = tf;
     l.threads = g.threads;
     l.grid = grid_for_pass(p, g);
     l.phase = 0;
@@ -511,98 +651,105 @@ static int mcclean_schedule(qg_plan* p, const int32_t* axes, const double* angle
     d.part = p->d_part;
     return d;
   };
+  auto gates = [&](PassDesc& d, const Geo& G, int i, double sign, bool own) {
+    for (int lb = G.own_lo; lb < G.own_hi; ++lb) {
+      const int b = (lb < G.c) ? lb : G.p + (lb - G.c);
+      const int q = n - 1 - b;
+      set_gate(d, lb, axes[i * n + q], angles[i * n + q], sign);
+      if (own) d.own[lb] = 1;
+    }
+  };
 
   // optional ini_state preparation: RY(pi/4) on every qubit of the given vector
-  const void* prev = nullptr;
+  const void* cur = nullptr;
   if (with_ini) {
     const void* src = wsp(p, p->off_ini);
     for (int g = 0; g < P; ++g) {
       const Geo& G = p->geo[g];
-      PassDesc& d = push(G, false);
+      PassDesc& d = push(G, false, ROLE_FWD);
       d.flags = F_LOAD_A | F_ROT_A | F_STORE_A;
       d.a_in = src;
       d.a_out = wsp(p, p->off_prep);
       for (int lb = G.own_lo; lb < G.own_hi; ++lb) set_gate(d, lb, QG_AXIS_Y, M_PI / 4, 1.0);
       src = wsp(p, p->off_prep);
     }
-    prev = wsp(p, p->off_prep);
+    cur = wsp(p, p->off_prep);
   }
-  // forward
+  // forward sweep
   for (int i = 0; i < L; ++i) {
     for (int g = 0; g < P; ++g) {
       const Geo& G = p->geo[g];
       const bool last = (i == L - 1) && (g == P - 1);
       const bool obs_fused = last && diag_obs;
-      PassDesc& d = push(G, obs_fused);
+      PassDesc& d = push(G, obs_fused, obs_fused ? ROLE_TWO_FP64 : ROLE_FWD);
       uint32_t f = F_ROT_A | F_STORE_A;
       if (g == 0) {
         f |= ent_flag_fwd;
-        f |= prev ? F_LOAD_A : F_INIT_TAB;
+        f |= cur ? F_LOAD_A : F_INIT_TAB;
       } else {
         f |= F_LOAD_A;
+      }
+      void* out;
+      if (p->history) {
+        out = hist(i, g);
+      } else if (g == P - 1 && slot[i] >= 0) {
+        out = ckpt(i);                       // checkpointed layer end
+      } else if (g == 0) {
+        out = other(cur);                    // the low pass gathers: out of place
+      } else {
+        out = (void*)cur;                    // high passes work in place
       }
       if (obs_fused) {
         f |= F_OBS_DIAG | F_STORE_B;
         d.b_out = lam[0];
         d.row_base = row;
-        fin_rows[0] = row + G.K;
         row += kMaxK + 1;
       }
       d.flags = f;
-      d.a_in = prev;
-      d.a_out = hist(i, g);
-      for (int lb = G.own_lo; lb < G.own_hi; ++lb) {
-        const int b = (lb < G.c) ? lb : G.p + (lb - G.c);
-        const int q = n - 1 - b;
-        set_gate(d, lb, axes[i * n + q], angles[i * n + q], 1.0);
-      }
-      prev = hist(i, g);
+      d.a_in = cur;
+      d.a_out = out;
+      gates(d, G, i, 1.0, false);
+      cur = out;
     }
   }
-  p->last_state_ptr = hist(L - 1, P - 1);
+  p->last_state_ptr = (void*)cur;
   if (!grad) {
     ndesc_out = nd;
-    std::vector<int> ptr(2), rows;
-    if (fin_rows[0] >= 0) rows.push_back(fin_rows[0]);
-    ptr[0] = 0;
-    ptr[1] = (int)rows.size();
-    (void)ptr;
     return QG_OK;
   }
-  // reverse
-  int cur = 0;
+  // adjoint sweep
+  int lc = 0;
+  const void* acur = cur;
   for (int i = L - 1; i >= 0; --i) {
     for (int g = P - 1; g >= 0; --g) {
       const Geo& G = p->geo[g];
-      PassDesc& d = push(G, true);
+      PassDesc& d = push(G, true, p->history ? ROLE_REV_HIST : ROLE_TWO_FP64);
       uint32_t f = F_LOAD_A | F_ROT_A | F_LOAD_B | F_ROT_B | F_STORE_B | F_REDUCE;
-      d.a_in = hist(i, g);
-      d.b_in = lam[cur];
-      if (g == 0) {
-        f |= ent_flag_rev;
-        if (p->ent == QG_ENT_CNOT) {
-          d.b_out = lam[cur ^ 1];
-          cur ^= 1;
-        } else {
-          d.b_out = lam[cur];
-        }
+      d.b_in = lam[lc];
+      if (g == 0 && p->ent == QG_ENT_CNOT) {
+        d.b_out = lam[lc ^ 1];
+        lc ^= 1;
       } else {
-        d.b_out = lam[cur];
+        d.b_out = lam[lc];
       }
+      if (p->history) {
+        d.a_in = hist(i, g);
+      } else {
+        const void* ain = (g == P - 1 && slot[i] >= 0) ? ckpt(i) : acur;
+        void* aout = (g == 0) ? other(ain) : (void*)ain;
+        d.a_in = ain;
+        d.a_out = aout;
+        f |= F_STORE_A;
+        acur = aout;
+      }
+      if (g == 0) f |= ent_flag_rev;
       d.flags = f;
       d.row_base = row;
-      for (int lb = G.own_lo; lb < G.own_hi; ++lb) {
-        const int b = (lb < G.c) ? lb : G.p + (lb - G.c);
-        const int q = n - 1 - b;
-        set_gate(d, lb, axes[i * n + q], angles[i * n + q], -1.0);
-        d.own[lb] = 1;
-        fin_rows[1 + i * n + q] = row + lb;
-      }
+      gates(d, G, i, -1.0, true);
       row += kMaxK + 1;
     }
   }
   ndesc_out = nd;
-  (void)row_cnt;
   return QG_OK;
 }
 
@@ -617,6 +764,13 @@ static int upload_finalize(qg_plan* p, const std::vector<std::vector<int>>& out_
   const int rows_off = nout + 1;
   const int cnt_off = (int)buf.size();
   for (int r = 0; r < p->nrows; ++r) buf.push_back(r < (int)cnt.size() ? cnt[r] : 0);
+  if (buf.size() > p->fin_cap) {
+    HIPCHK(hipStreamSynchronize(p->ctx->stream));
+    hipFree(p->d_fin);
+    p->d_fin = nullptr;
+    HIPCHK(hipMalloc(&p->d_fin, sizeof(int) * buf.size()));
+    p->fin_cap = buf.size();
+  }
   HIPCHK(hipMemcpyAsync(p->d_fin, buf.data(), sizeof(int) * buf.size(), hipMemcpyHostToDevice, p->ctx->stream));
   HIPCHK(hipMemcpyAsync(p->d_scale, scale.data(), sizeof(double) * nout, hipMemcpyHostToDevice, p->ctx->stream));
   HIPCHK(launch_finalize(nout, p->d_fin, p->d_fin + rows_off, p->d_fin + cnt_off, p->d_scale, p->d_part, p->gmax,
@@ -691,6 +845,7 @@ static int mcclean_run(qg_plan* p, const int32_t* axes, const double* angles, co
   std::vector<double> scale(out_rows.size(), 1.0);
   CHK(run_launches(p, nd));
   HIPCHK(hipEventRecord(c->ev1, c->stream));
+  CHK(collect_profile(p));
   CHK(upload_finalize(p, out_rows, cnt, scale));
   HIPCHK(hipEventElapsedTime(&p->last_ms, c->ev0, c->ev1));
   const int nout = (int)out_rows.size();
@@ -712,18 +867,274 @@ extern "C" int qg_mcclean_expec_val(qg_plan* p, const int32_t* axes, const doubl
 }
 
 // ---------------------------------------------------------------------------
-// QAOA (not yet available)
+// QAOA (qradient/circuit_logic/qaoa.py:40-70), batched over circuits
 // ---------------------------------------------------------------------------
-extern "C" int qg_qaoa_plan(qg_ctx*, int, int, int, int, qg_plan**) {
-  return fail(QG_ERR_UNSUPPORTED, "QAOA plans are not available in this build");
+extern "C" int qg_qaoa_plan(qg_ctx* ctx, int n, int p_layers, int dtype, int batch, qg_plan** out) {
+  if (!ctx || !out) return fail(QG_ERR_INVALID, "NULL argument");
+  if (n < 1 || n > kMaxQubits) return fail(QG_ERR_INVALID, "qubit number must be in [1, 32]");
+  if (p_layers < 1) return fail(QG_ERR_INVALID, "layer number must be >= 1");
+  if (dtype != QG_C128 && dtype != QG_C64) return fail(QG_ERR_INVALID, "dtype must be QG_C128 or QG_C64");
+  if (batch < 1) return fail(QG_ERR_INVALID, "batch must be >= 1");
+  HIPCHK(hipSetDevice(ctx->device));
+  qg_plan* p = new qg_plan();
+  p->ctx = ctx;
+  p->kind = PK_QAOA;
+  p->n = n;
+  p->L = p_layers;
+  p->dtype = dtype;
+  p->batch = batch;
+  p->geo = layer_geometry(n, p->kmax, p->cmin);
+  const int P = (int)p->geo.size();
+  p->S = (size_t)(1ull << n) * (dtype == QG_C64 ? 8 : 16);
+  const size_t per_circuit = (size_t)(P * p_layers + 2) * p->S;
+  size_t free_b = 0, total_b = 0;
+  hipMemGetInfo(&free_b, &total_b);
+  const size_t budget = (size_t)(0.80 * (double)free_b);
+  if (per_circuit > budget) {
+    delete p;
+    return fail(QG_ERR_UNSUPPORTED, "one QAOA circuit's state history does not fit in device memory");
+  }
+  p->chunk = (int)std::min<size_t>((size_t)batch, budget / per_circuit);
+  p->off_hist = 0;
+  p->off_lam = (size_t)p->chunk * P * p_layers * p->S;
+  p->ws_bytes = p->off_lam + (size_t)p->chunk * 2 * p->S;
+  for (const Geo& g : p->geo)
+    for (int role = 0; role < 3; ++role) {
+      bool tf;
+      p->gmax = std::max(p->gmax, grid_for_pass(p, with_role(g, dtype, role, &tf)));
+    }
+  const int ndesc = p->chunk * 2 * P * p_layers;
+  p->nrows = p->chunk * (P * p_layers + 1) * (kMaxK + 1);
+  int r = alloc_small(p, ndesc, p->nrows, p->chunk * (1 + 2 * p_layers));
+  if (r != QG_OK) {
+    qg_plan_destroy(p);
+    return r;
+  }
+  hipFree(p->d_terms);
+  p->d_terms = nullptr;
+  p->max_terms = n + n * (n - 1) / 2;
+  hipError_t e = hipMalloc(&p->d_terms, sizeof(DTerm) * (size_t)p->max_terms * p->chunk);
+  std::vector<double> init(n + 1, std::pow(2.0, -0.5 * n));
+  if (e == hipSuccess) e = hipMemcpy(p->d_init, init.data(), sizeof(double) * (n + 1), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    qg_plan_destroy(p);
+    return fail(QG_ERR_HIP, std::string("qaoa plan tables: ") + hipGetErrorString(e));
+  }
+  *out = p;
+  return QG_OK;
 }
-extern "C" int qg_qaoa_grad_run(qg_plan*, int, const qg_term*, const int32_t*, const double*, const double*, double*,
-                                double*) {
-  return fail(QG_ERR_UNSUPPORTED, "QAOA plans are not available in this build");
+
+static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term_off, const double* betas,
+                    const double* gammas, double* E_out, double* grad_out, bool grad) {
+  if (!p || p->kind != PK_QAOA) return fail(QG_ERR_INVALID, "not a QAOA plan");
+  if (B < 1 || B > p->batch) return fail(QG_ERR_INVALID, "B must be in [1, batch]");
+  if (!term_off || !betas || !gammas || !E_out || (grad && !grad_out)) return fail(QG_ERR_INVALID, "NULL argument");
+  const int n = p->n, L = p->L, P = (int)p->geo.size();
+  if (term_off[0] != 0) return fail(QG_ERR_INVALID, "term_off[0] must be 0");
+  for (int b = 0; b < B; ++b)
+    if (term_off[b + 1] < term_off[b] || term_off[b + 1] - term_off[b] > p->max_terms)
+      return fail(QG_ERR_INVALID, "bad term offsets");
+  for (int k = 0; k < B * L; ++k)
+    if (!std::isfinite(betas[k]) || !std::isfinite(gammas[k])) return fail(QG_ERR_INVALID, "non-finite parameter");
+  // split and classify the classical Hamiltonians
+  std::vector<ObsSplit> obs(B);
+  bool integral = true;
+  double W = 0.0;
+  for (int b = 0; b < B; ++b) {
+    CHK(split_observable(n, terms + term_off[b], term_off[b + 1] - term_off[b], obs[b]));
+    if (!obs[b].xy.empty())
+      return fail(QG_ERR_INVALID, "the QAOA classical Hamiltonian takes z / zz terms only (state.py:268-271)");
+    double wsum = 0.0;
+    for (const DTerm& t : obs[b].diag) {
+      wsum += std::fabs(t.w);
+      if (t.w != std::rint(t.w)) integral = false;
+    }
+    W = std::max(W, wsum);
+  }
+  const int phase = (integral && W <= 4096.0) ? 1 : 2;
+  const int Wi = (int)W;
+  qg_ctx* c = p->ctx;
+  HIPCHK(hipSetDevice(c->device));
+  CHK(ensure_workspace(p));
+  const int chunk = p->chunk;
+  const int lut_len = 2 * Wi + 1;
+  if (phase == 1) {
+    const int need = chunk * 2 * L * lut_len * 2;
+    if (need > p->lut_len) {
+      hipFree(p->d_lut);
+      p->d_lut = nullptr;
+      HIPCHK(hipMalloc(&p->d_lut, sizeof(double) * need));
+      p->lut_len = need;
+    }
+  }
+  const int per_out = 1 + 2 * L;
+  float total_ms = 0.f;
+  for (int c0 = 0; c0 < B; c0 += chunk) {
+    const int nb = std::min(chunk, B - c0);
+    // terms and phase tables of this chunk
+    std::vector<DTerm> tbuf((size_t)p->max_terms * nb);
+    std::vector<int> toff(nb), tcnt(nb);
+    for (int j = 0; j < nb; ++j) {
+      toff[j] = j * p->max_terms;
+      tcnt[j] = (int)obs[c0 + j].diag.size();
+      std::copy(obs[c0 + j].diag.begin(), obs[c0 + j].diag.end(), tbuf.begin() + toff[j]);
+    }
+    HIPCHK(hipMemcpy(p->d_terms, tbuf.data(), sizeof(DTerm) * tbuf.size(), hipMemcpyHostToDevice));
+    std::vector<double> lut;
+    if (phase == 1) {
+      lut.resize((size_t)nb * 2 * L * lut_len * 2);
+      for (int j = 0; j < nb; ++j)
+        for (int i = 0; i < L; ++i)
+          for (int dir = 0; dir < 2; ++dir) {
+            const double g = (dir == 0 ? 1.0 : -1.0) * gammas[(c0 + j) * L + i];
+            double* t = lut.data() + (((size_t)j * L + i) * 2 + dir) * lut_len * 2;
+            for (int k = 0; k < lut_len; ++k) {
+              const double dv = (double)(k - Wi);
+              t[2 * k] = std::cos(g * dv);
+              t[2 * k + 1] = -std::sin(g * dv);
+            }
+          }
+      HIPCHK(hipMemcpy(p->d_lut, lut.data(), sizeof(double) * lut.size(), hipMemcpyHostToDevice));
+    }
+    auto lut_ptr = [&](int j, int i, int dir) -> const double* {
+      return phase == 1 ? p->d_lut + (((size_t)j * L + i) * 2 + dir) * lut_len * 2 : nullptr;
+    };
+    auto hist = [&](int j, int i, int g) {
+      return wsp(p, p->off_hist + (((size_t)j * L + i) * P + g) * p->S);
+    };
+    auto lam = [&](int j) { return wsp(p, p->off_lam + (size_t)j * 2 * p->S); };
+    // descriptors: one batched launch per pass, nb consecutive descriptors
+    p->launches.clear();
+    int nd = 0, row = 0;
+    std::vector<std::vector<int>> out_rows((size_t)nb * per_out);
+    std::vector<int> cnt(p->nrows, 0);
+    Geo cur_geo;
+    auto begin_pass = [&](const Geo& G0, bool two, int role) {
+      bool tf = false;
+      const Geo G = with_role(G0, p->dtype, role, &tf);
+      cur_geo = G;
+      Launch l;
+      l.desc = nd;
+      l.batch = nb;
+      l.R = G.R;
+      l.K = G.K;
+      l.ta_float = tf;
+      l.threads = G.threads;
+      l.grid = grid_for_pass(p, G);
+      l.phase = phase;
+      l.two = two;
+      l.kind = 0;
+      p->launches.push_back(l);
+      return l.grid;
+    };
+    auto base_desc = [&](const Geo& G, int j) -> PassDesc& {
+      PassDesc& d = p->h_desc[nd++];
+      desc_geo(d, n, G);
+      d.grid = p->gmax;
+      d.init_tab = p->d_init;
+      d.dterms = p->d_terms + toff[j];
+      d.ndterms = tcnt[j];
+      d.lut_off = Wi;
+      d.part = p->d_part;
+      return d;
+    };
+    const uint32_t phase_flag = phase == 1 ? F_PHASE_LUT : 0u;
+    // forward
+    for (int i = 0; i < L; ++i) {
+      for (int g = 0; g < P; ++g) {
+        const Geo& G = p->geo[g];
+        const bool last = (i == L - 1) && (g == P - 1);
+        const int gr = begin_pass(G, last, last ? ROLE_TWO_FP64 : ROLE_FWD);
+        for (int j = 0; j < nb; ++j) {
+          PassDesc& d = base_desc(cur_geo, j);
+          uint32_t f = F_ROT_A | F_STORE_A | phase_flag;
+          if (g == 0) {
+            f |= (i == 0 ? F_INIT_TAB : F_LOAD_A) | F_PHASE_PRE;
+            d.phase_gamma = gammas[(c0 + j) * L + i];
+            d.lut = lut_ptr(j, i, 0);
+          } else {
+            f |= F_LOAD_A;
+          }
+          d.a_in = (i == 0 && g == 0) ? nullptr : (g == 0 ? hist(j, i - 1, P - 1) : hist(j, i, g - 1));
+          d.a_out = hist(j, i, g);
+          for (int lb = G.own_lo; lb < G.own_hi; ++lb) set_gate(d, lb, QG_AXIS_X, betas[(c0 + j) * L + i], 1.0);
+          if (last) {
+            f |= F_OBS_DIAG | F_STORE_B;
+            d.b_out = lam(j);
+            d.row_base = row;
+            out_rows[(size_t)j * per_out].push_back(row + G.K);
+            for (int k = 0; k <= kMaxK; ++k) cnt[row + k] = gr;
+            row += kMaxK + 1;
+          }
+          d.flags = f;
+        }
+      }
+    }
+    if (grad) {
+      for (int i = L - 1; i >= 0; --i) {
+        for (int g = P - 1; g >= 0; --g) {
+          const Geo& G = p->geo[g];
+          const int gr = begin_pass(G, true, ROLE_REV_HIST);
+          for (int j = 0; j < nb; ++j) {
+            PassDesc& d = base_desc(cur_geo, j);
+            uint32_t f = F_LOAD_A | F_ROT_A | F_LOAD_B | F_ROT_B | F_STORE_B | F_REDUCE | phase_flag;
+            d.a_in = hist(j, i, g);
+            d.b_in = lam(j);
+            d.b_out = lam(j);
+            d.row_base = row;
+            for (int lb = G.own_lo; lb < G.own_hi; ++lb) {
+              set_gate(d, lb, QG_AXIS_X, betas[(c0 + j) * L + i], -1.0);
+              d.own[lb] = 1;
+              out_rows[(size_t)j * per_out + 1 + 2 * i].push_back(row + lb);
+            }
+            if (g == 0) {
+              f |= F_DRED_POST | F_PHASE_POST;
+              d.phase_gamma = -gammas[(c0 + j) * L + i];
+              d.lut = lut_ptr(j, i, 1);
+              out_rows[(size_t)j * per_out + 2 + 2 * i].push_back(row + G.K);
+            }
+            d.flags = f;
+            for (int k = 0; k <= kMaxK; ++k) cnt[row + k] = gr;
+            row += kMaxK + 1;
+          }
+        }
+      }
+    }
+    std::vector<double> scale(out_rows.size(), 1.0);
+    if (!grad) {
+      for (int j = 0; j < nb; ++j)
+        for (int k = 1; k < per_out; ++k) out_rows[(size_t)j * per_out + k].clear();
+    }
+    CHK(run_launches(p, nd));
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    CHK(collect_profile(p));
+    CHK(upload_finalize(p, out_rows, cnt, scale));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    total_ms += ms;
+    const int nout = nb * per_out;
+    HIPCHK(hipMemcpyAsync(p->h_out, p->d_out, sizeof(double) * nout, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int j = 0; j < nb; ++j) {
+      const double* o = p->h_out + (size_t)j * per_out;
+      E_out[c0 + j] = o[0];
+      if (grad)
+        for (int k = 0; k < 2 * L; ++k) grad_out[(size_t)(c0 + j) * 2 * L + k] = o[1 + k];
+    }
+    p->last_state_ptr = hist(0, L - 1, P - 1);
+  }
+  p->last_ms = total_ms;
+  return QG_OK;
 }
-extern "C" int qg_qaoa_expec_val(qg_plan*, int, const qg_term*, const int32_t*, const double*, const double*,
-                                 double*) {
-  return fail(QG_ERR_UNSUPPORTED, "QAOA plans are not available in this build");
+
+extern "C" int qg_qaoa_grad_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term_off,
+                                const double* betas, const double* gammas, double* E_out, double* grad_out) {
+  return qaoa_run(p, B, terms, term_off, betas, gammas, E_out, grad_out, true);
+}
+
+extern "C" int qg_qaoa_expec_val(qg_plan* p, int B, const qg_term* terms, const int32_t* term_off,
+                                 const double* betas, const double* gammas, double* E_out) {
+  return qaoa_run(p, B, terms, term_off, betas, gammas, E_out, nullptr, false);
 }
 
 // ---------------------------------------------------------------------------
@@ -964,5 +1375,30 @@ extern "C" int qg_state_scale(qg_state* s, double re, double im) {
   HIPCHK(hipSetDevice(s->ctx->device));
   HIPCHK(simple_scale(s->dtype, s->v, s->n, re, im, s->ctx->stream));
   HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  return QG_OK;
+}
+
+extern "C" int qg_plan_tune(qg_plan* p, int kmax, int cmin, int r_fwd, int r_rev) {
+  if (!p) return fail(QG_ERR_INVALID, "NULL plan");
+  if (kmax < 4 || kmax > kMaxK) return fail(QG_ERR_INVALID, "kmax must be in [4, 13]");
+  if (cmin < 0 || cmin >= kmax) return fail(QG_ERR_INVALID, "cmin must be in [0, kmax)");
+  if (r_fwd < 4 || r_fwd > 5 || r_rev < 4 || r_rev > 5) return fail(QG_ERR_INVALID, "r_fwd, r_rev must be 4 or 5");
+  if (p->kind != PK_MCCLEAN) return fail(QG_ERR_UNSUPPORTED, "tuning applies to McClean plans");
+  HIPCHK(hipSetDevice(p->ctx->device));
+  HIPCHK(hipStreamSynchronize(p->ctx->stream));
+  const size_t old_ws = p->ws_bytes;
+  p->kmax = kmax;
+  p->cmin = cmin;
+  p->r_fwd = r_fwd;
+  p->r_rev = r_rev;
+  free_small(p);
+  CHK(mcclean_layout(p));
+  if (p->ws && p->ws_bytes > old_ws) {
+    if (!p->ws_owned) return fail(QG_ERR_INVALID, "tuned layout needs a larger workspace than the one supplied");
+    hipFree(p->ws);
+    p->ws = nullptr;
+    p->ws_owned = false;
+  }
+  p->last_state_ptr = nullptr;
   return QG_OK;
 }

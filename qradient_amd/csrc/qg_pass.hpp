@@ -1,7 +1,7 @@
-// HIP kernels for the statevector grad_run path on CDNA4 (gfx950).
+// Tile-pass engine for the statevector grad_run path on CDNA4 (gfx950).
 //
 // A statevector of 2^n amplitudes is processed in "passes".  Each pass views
-// the index bits as K local bits and n-K tile bits; one workgroup handles one
+// the index bits as K local bits and n-K tile bits; a workgroup handles one
 // tile (2^K amplitudes) at a time:
 //
 //   natural layout : thread bits = local bits [0, t), register bits = [t, K)
@@ -10,10 +10,13 @@
 //                    lo_s = max(0, t - R*s); reached by an LDS transpose
 //
 // Every single-qubit gate of the pass is applied to the register bits of the
-// stage in which its bit first appears, so all K gates of a tile are applied
-// with ceil(t/R) + 1 LDS round trips and one HBM read + write per state.
-// Amplitudes are held in FP64 in registers and LDS whatever the storage type,
-// so a complex64 state is rounded once per pass, not once per gate.
+// stage in which its bit first appears, so all K gates of a tile cost
+// ceil(t/R) + 1 LDS round trips and one HBM read + write per state.
+// Arithmetic is FP64 in registers for the propagated state; LDS holds the
+// storage type, so a complex64 state is rounded at each LDS transpose and at
+// the store (a few times per pass, never per gate).  In an adjoint pass the
+// read-only companion state (psi from the history) may be held in FP32
+// registers: it only enters the FP64-accumulated reductions.
 //
 // The CNOT ladder of the McClean circuit (reference
 // qradient/physical_components/state.py:209-251) is a GF(2)-linear map of the
@@ -21,60 +24,126 @@
 // applied as an address map on the global load (forward) or store (reverse)
 // of the low-bit pass: no pass of its own.
 //
-// Reductions Im<B|P_q|A> (reference grad entries, mc_clean.py:68-76) are
-// accumulated in FP64 per thread, reduced per workgroup in a fixed order and
-// written to partial rows; k_finalize sums the rows in a fixed order, so
-// results are bitwise reproducible for a given plan.
+// Reductions Im<B|P_q|A> (the reference's gradient entries,
+// mc_clean.py:68-76) are accumulated in FP64, summed per wave and per
+// workgroup in a fixed order and written to partial rows; k_finalize sums the
+// rows in a fixed order, so results are bitwise reproducible for a plan.
+#pragma once
 #include <hip/hip_runtime.h>
+
 #include <type_traits>
 #include <utility>
+
 #include "qg_internal.h"
 
 namespace qg {
 
-struct Cd {
-  double re, im;
+template <class T>
+struct Cx {
+  T re, im;
 };
+typedef Cx<double> Cd;
 
-// Global (address space 1) views: loads and stores through them compile to
-// global_load / global_store instead of flat accesses.
 typedef float f2v __attribute__((ext_vector_type(2)));
 typedef double d2v __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(1))) f2v gf2v;
 typedef __attribute__((address_space(1))) d2v gd2v;
+typedef __attribute__((address_space(3))) f2v lf2v;
+typedef __attribute__((address_space(3))) d2v ld2v;
 
+// Global (address space 1) loads/stores compile to global_load/global_store.
 template <class S>
 struct Stor;
 template <>
 struct Stor<float> {
-  static __device__ __forceinline__ Cd ld(const void* p, uint32_t i) {
-    const f2v v = ((const gf2v*)p)[i];
-    return {(double)v.x, (double)v.y};
+  typedef f2v V;
+  // Amplitude i of the state at p.  For states under 4 GiB the byte offset is
+  // formed in 32 bits, so the address is a scalar base plus one VGPR.
+  template <class T>
+  static __device__ __forceinline__ Cx<T> ld(const void* p, uint32_t i) {
+    const f2v v = *(const gf2v*)((const char*)p + (uint32_t)(i << 3));
+    return {(T)v.x, (T)v.y};
   }
-  static __device__ __forceinline__ void st(void* p, uint32_t i, Cd v) {
+  template <class T>
+  static __device__ __forceinline__ Cx<T> ld_big(const void* p, uint32_t i) {
+    const f2v v = ((const gf2v*)p)[i];
+    return {(T)v.x, (T)v.y};
+  }
+  template <class T>
+  static __device__ __forceinline__ void st(void* p, uint32_t i, Cx<T> v) {
+    f2v w;
+    w.x = (float)v.re;
+    w.y = (float)v.im;
+    *(gf2v*)((char*)p + (uint32_t)(i << 3)) = w;
+  }
+  template <class T>
+  static __device__ __forceinline__ void st_big(void* p, uint32_t i, Cx<T> v) {
     f2v w;
     w.x = (float)v.re;
     w.y = (float)v.im;
     ((gf2v*)p)[i] = w;
   }
+  template <class T>
+  static __device__ __forceinline__ void lds_st(void* lds, uint32_t i, Cx<T> v) {
+    f2v w;
+    w.x = (float)v.re;
+    w.y = (float)v.im;
+    ((lf2v*)lds)[i] = w;
+  }
+  template <class T>
+  static __device__ __forceinline__ Cx<T> lds_ld(const void* lds, uint32_t i) {
+    const f2v v = ((const lf2v*)lds)[i];
+    return {(T)v.x, (T)v.y};
+  }
 };
 template <>
 struct Stor<double> {
-  static __device__ __forceinline__ Cd ld(const void* p, uint32_t i) {
-    const d2v v = ((const gd2v*)p)[i];
-    return {v.x, v.y};
+  typedef d2v V;
+  template <class T>
+  static __device__ __forceinline__ Cx<T> ld(const void* p, uint32_t i) {
+    const d2v v = *(const gd2v*)((const char*)p + (uint32_t)(i << 4));
+    return {(T)v.x, (T)v.y};
   }
-  static __device__ __forceinline__ void st(void* p, uint32_t i, Cd v) {
+  template <class T>
+  static __device__ __forceinline__ Cx<T> ld_big(const void* p, uint32_t i) {
+    const d2v v = ((const gd2v*)p)[i];
+    return {(T)v.x, (T)v.y};
+  }
+  template <class T>
+  static __device__ __forceinline__ void st(void* p, uint32_t i, Cx<T> v) {
     d2v w;
-    w.x = v.re;
-    w.y = v.im;
+    w.x = (double)v.re;
+    w.y = (double)v.im;
+    *(gd2v*)((char*)p + (uint32_t)(i << 4)) = w;
+  }
+  template <class T>
+  static __device__ __forceinline__ void st_big(void* p, uint32_t i, Cx<T> v) {
+    d2v w;
+    w.x = (double)v.re;
+    w.y = (double)v.im;
     ((gd2v*)p)[i] = w;
+  }
+  template <class T>
+  static __device__ __forceinline__ void lds_st(void* lds, uint32_t i, Cx<T> v) {
+    d2v w;
+    w.x = (double)v.re;
+    w.y = (double)v.im;
+    ((ld2v*)lds)[i] = w;
+  }
+  template <class T>
+  static __device__ __forceinline__ Cx<T> lds_ld(const void* lds, uint32_t i) {
+    const d2v v = ((const ld2v*)lds)[i];
+    return {(T)v.x, (T)v.y};
   }
 };
 
-// LDS slot of local index j: XOR swizzle so that strided stage layouts stay
-// free of bank conflicts for 16-byte accesses.
-__device__ __forceinline__ uint32_t swz(uint32_t j) { return j ^ ((j >> 4) & 15u); }
+// LDS slot of local index j: an XOR swizzle that keeps every stage layout
+// free of bank conflicts (64 banks x 4 B: 32 slots of 8 B or 16 of 16 B per row).
+template <class S>
+__device__ __forceinline__ uint32_t swz(uint32_t j) {
+  if constexpr (sizeof(S) == 4) return j ^ ((j >> 5) & 31u);
+  else return j ^ ((j >> 4) & 15u);
+}
 
 // Local index held by (thread tid, register a) when register bits sit at [lo, lo+R).
 template <int R>
@@ -115,14 +184,15 @@ __device__ __forceinline__ double cz_sign(uint32_t x, int n) {
 // 2x2 gates on register bit KK: U = c*I - i*s*P for P = X, Y, Z (reference
 // state.py:90-92, 142-144, 168-170).  The planner negates s for inverses.
 // ---------------------------------------------------------------------------
-template <int R, int KK>
-__device__ __forceinline__ void gate_bit(Cd (&x)[1 << R], int type, double c, double s) {
+template <int R, int KK, class T>
+__device__ __forceinline__ void gate_bit(Cx<T> (&x)[1 << R], int type, double cd, double sd) {
   constexpr int A = 1 << R;
+  const T c = (T)cd, s = (T)sd;
   if (type == G_X) {
 #pragma unroll
     for (int a = 0; a < A; ++a) {
       if (a & (1 << KK)) continue;
-      const Cd x0 = x[a], x1 = x[a | (1 << KK)];
+      const Cx<T> x0 = x[a], x1 = x[a | (1 << KK)];
       // x0' = c x0 - i s x1 ; x1' = c x1 - i s x0
       x[a] = {c * x0.re + s * x1.im, c * x0.im - s * x1.re};
       x[a | (1 << KK)] = {c * x1.re + s * x0.im, c * x1.im - s * x0.re};
@@ -131,7 +201,7 @@ __device__ __forceinline__ void gate_bit(Cd (&x)[1 << R], int type, double c, do
 #pragma unroll
     for (int a = 0; a < A; ++a) {
       if (a & (1 << KK)) continue;
-      const Cd x0 = x[a], x1 = x[a | (1 << KK)];
+      const Cx<T> x0 = x[a], x1 = x[a | (1 << KK)];
       // -iY = [[0,-1],[1,0]]: x0' = c x0 - s x1 ; x1' = c x1 + s x0
       x[a] = {c * x0.re - s * x1.re, c * x0.im - s * x1.im};
       x[a | (1 << KK)] = {c * x1.re + s * x0.re, c * x1.im + s * x0.im};
@@ -139,23 +209,32 @@ __device__ __forceinline__ void gate_bit(Cd (&x)[1 << R], int type, double c, do
   } else if (type == G_Z) {
 #pragma unroll
     for (int a = 0; a < A; ++a) {
-      const Cd v = x[a];
+      const Cx<T> v = x[a];
       // bit 0: (c - i s) v ; bit 1: (c + i s) v
-      const double ss = (a & (1 << KK)) ? s : -s;
+      const T ss = (a & (1 << KK)) ? s : -s;
       x[a] = {c * v.re - ss * v.im, c * v.im + ss * v.re};
     }
   }
 }
 
-// sum over the register pairs of bit KK of Im<l|P|p>.
-template <int R, int KK>
-__device__ __forceinline__ double red_bit(const Cd (&l)[1 << R], const Cd (&p)[1 << R], int type) {
+template <class T>
+__device__ __forceinline__ double imcj(const Cd& l, const Cx<T>& p) {  // Im(conj(l) p)
+  return l.re * (double)p.im - l.im * (double)p.re;
+}
+template <class T>
+__device__ __forceinline__ double recj(const Cd& l, const Cx<T>& p) {  // Re(conj(l) p)
+  return l.re * (double)p.re + l.im * (double)p.im;
+}
+
+// sum over the register pairs of bit KK of Im<l|P|p>, in FP64.
+template <int R, int KK, class T>
+__device__ __forceinline__ double red_bit(const Cd (&l)[1 << R], const Cx<T> (&p)[1 << R], int type) {
   constexpr int A = 1 << R;
   double acc = 0.0;
   if (type == G_Z) {
 #pragma unroll
     for (int a = 0; a < A; ++a) {
-      const double v = l[a].re * p[a].im - l[a].im * p[a].re;
+      const double v = imcj(l[a], p[a]);
       acc += (a & (1 << KK)) ? -v : v;
     }
   } else if (type == G_X) {
@@ -163,30 +242,27 @@ __device__ __forceinline__ double red_bit(const Cd (&l)[1 << R], const Cd (&p)[1
     for (int a = 0; a < A; ++a) {
       if (a & (1 << KK)) continue;
       const int b = a | (1 << KK);
-      acc += (l[a].re * p[b].im - l[a].im * p[b].re) + (l[b].re * p[a].im - l[b].im * p[a].re);
+      acc += imcj(l[a], p[b]) + imcj(l[b], p[a]);
     }
   } else if (type == G_Y) {
 #pragma unroll
     for (int a = 0; a < A; ++a) {
       if (a & (1 << KK)) continue;
       const int b = a | (1 << KK);
-      acc += -(l[a].re * p[b].re + l[a].im * p[b].im) + (l[b].re * p[a].re + l[b].im * p[a].im);
+      acc += recj(l[b], p[a]) - recj(l[a], p[b]);
     }
   }
   return acc;
 }
 
-template <int R>
-__device__ __forceinline__ void lds_transpose(Cd (&x)[1 << R], double2* lds, uint32_t tid, int lo_from, int lo_to) {
+template <class S, int R, class T>
+__device__ __forceinline__ void lds_transpose(Cx<T> (&x)[1 << R], void* lds, uint32_t tid, int lo_from, int lo_to) {
   constexpr int A = 1 << R;
 #pragma unroll
-  for (int a = 0; a < A; ++a) lds[swz(lidx<R>(tid, a, lo_from))] = make_double2(x[a].re, x[a].im);
+  for (int a = 0; a < A; ++a) Stor<S>::lds_st(lds, swz<S>(lidx<R>(tid, a, lo_from)), x[a]);
   __syncthreads();
 #pragma unroll
-  for (int a = 0; a < A; ++a) {
-    const double2 v = lds[swz(lidx<R>(tid, a, lo_to))];
-    x[a] = {v.x, v.y};
-  }
+  for (int a = 0; a < A; ++a) x[a] = Stor<S>::template lds_ld<T>(lds, swz<S>(lidx<R>(tid, a, lo_to)));
   __syncthreads();
 }
 
@@ -272,7 +348,10 @@ __device__ __forceinline__ Cd phase_of(const PassDesc* __restrict__ d, double D)
   }
 }
 
-__device__ __forceinline__ Cd cmul(Cd a, Cd b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+template <class T>
+__device__ __forceinline__ Cx<T> cmul(Cd a, Cx<T> b) {
+  return {(T)(a.re * b.re - a.im * b.im), (T)(a.re * b.im + a.im * b.re)};
+}
 
 template <int I>
 using ic = std::integral_constant<int, I>;
@@ -291,19 +370,25 @@ __device__ __forceinline__ double wave_sum(double v, int width) {
   return v;
 }
 
-// ---------------------------------------------------------------------------
-// One pass over the tiles assigned to this workgroup.
-// ---------------------------------------------------------------------------
 __device__ __forceinline__ int nnew_of(const int* los, int s, int R) {
   if (s == 0) return R;
   const int d = los[s - 1] - los[s];
   return d < R ? d : R;
 }
 
-template <class S, int R, bool TWO, int PH>
-__device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, double2* lds, uint32_t tid_in,
-                                         uint32_t blk, uint32_t nblk) {
+// ---------------------------------------------------------------------------
+// One pass over the tiles assigned to this workgroup.
+//   S  : storage and LDS element type (float: complex64, double: complex128)
+//   R  : register bits per thread (2^R amplitudes)
+//   TWO: an adjoint pass with the companion state A and the propagated B
+//   PH : diagonal phase method (0 none, 1 table, 2 sincos)
+//   TA : register precision of A in a TWO pass (the propagated state is FP64)
+// ---------------------------------------------------------------------------
+template <class S, int R, bool TWO, int PH, class TA, bool BIG>
+__device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* lds, uint32_t tid_in, uint32_t blk,
+                                         uint32_t nblk) {
   constexpr int A = 1 << R;
+  typedef typename std::conditional<TWO, TA, double>::type TAr;
   const int n = d->n, K = d->K, t = d->t, c = d->c, p = d->p;
   const uint32_t flags = d->flags;
   const uint32_t ntiles = 1u << (n - K);
@@ -331,27 +416,25 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, double2
     rpos[k] = gpos_of_local(t + k, c, p);
     rbit[k] = 1u << rpos[k];
   }
-  const bool use_map = (flags & (F_GATHER | F_SCATTER)) != 0;
 
-  // Per-wave FP64 accumulators in LDS, after the tile buffer: [wave][K + 1].
-  // Each (wave, slot) has exactly one writer (lane 0), so the summation order
-  // is fixed.
-  double* wacc = reinterpret_cast<double*>(lds + (1u << K));
+  // Per-wave FP64 accumulators in LDS after the tile buffer: [wave][K + 1];
+  // each (wave, slot) has one writer (lane 0), so the summation order is fixed.
+  double* wacc = reinterpret_cast<double*>(reinterpret_cast<char*>(lds) + (sizeof(S) * 2u << K));
   const bool reduce = TWO && (flags & (F_REDUCE | F_DRED_POST | F_OBS_DIAG)) && d->row_base >= 0;
   if (reduce) {
     for (uint32_t i = tid_in; i < (uint32_t)(nwaves * (K + 1)); i += nthr) wacc[i] = 0.0;
     __syncthreads();
   }
 
-  // XCD-aware tile order: workgroup blk runs on XCD blk % 8; give each XCD a
+  // XCD-aware tile order: workgroup blk runs on XCD blk % 8; each XCD gets a
   // contiguous range of tiles so tiles that share cache lines share an L2.
   const bool xcd = (nblk >= 8) && (nblk % 8 == 0) && (ntiles % nblk == 0);
   const uint32_t per_x = ntiles / 8, slots = nblk / 8;
   const uint32_t rounds = (ntiles + nblk - 1) / nblk;
 
   for (uint32_t r = 0; r < rounds; ++r) {
-    // Recompute every thread-index-derived address inside the loop rather
-    // than hoisting loop-invariant LDS/global offsets into registers.
+    // Recompute thread-index-derived addresses inside the loop rather than
+    // hoisting loop-invariant offsets into registers.
     uint32_t tid = tid_in;
     asm volatile("" : "+v"(tid));
     uint32_t tt;
@@ -385,18 +468,21 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, double2
         adr[a] = v;
       }
     }
-    Cd xa[A], xb[A];
+    Cx<TAr> xa[A];
+    Cd xb[TWO ? A : 1];
     if (flags & F_INIT_TAB) {
 #pragma unroll
-      for (int a = 0; a < A; ++a) xa[a] = {d->init_tab[__popc(adr[a])], 0.0};
+      for (int a = 0; a < A; ++a) xa[a] = {(TAr)d->init_tab[__popc(adr[a])], (TAr)0};
     } else {
 #pragma unroll
-      for (int a = 0; a < A; ++a) xa[a] = Stor<S>::ld(d->a_in, adr[a]);
+      for (int a = 0; a < A; ++a)
+        xa[a] = BIG ? Stor<S>::template ld_big<TAr>(d->a_in, adr[a]) : Stor<S>::template ld<TAr>(d->a_in, adr[a]);
     }
-    if (TWO) {
+    if constexpr (TWO) {
       if (flags & F_LOAD_B) {
 #pragma unroll
-        for (int a = 0; a < A; ++a) xb[a] = Stor<S>::ld(d->b_in, adr[a]);
+        for (int a = 0; a < A; ++a)
+          xb[a] = BIG ? Stor<S>::template ld_big<double>(d->b_in, adr[a]) : Stor<S>::template ld<double>(d->b_in, adr[a]);
       } else {
 #pragma unroll
         for (int a = 0; a < A; ++a) xb[a] = {0.0, 0.0};
@@ -411,7 +497,7 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, double2
 #pragma unroll
         for (int k = 0; k < R; ++k)
           if (a & (1 << k)) x |= rbit[k];
-        const double sg = cz_sign(x, n);
+        const TAr sg = (TAr)cz_sign(x, n);
         xa[a] = {sg * xa[a].re, sg * xa[a].im};
       }
     }
@@ -428,11 +514,12 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, double2
 #pragma unroll 1
     for (int s = 0; s < ns; ++s) {
       if (s > 0) {
-        lds_transpose<R>(xa, lds, tid, los[s - 1], los[s]);
-        if (TWO) lds_transpose<R>(xb, lds, tid, los[s - 1], los[s]);
+        lds_transpose<S, R>(xa, lds, tid, los[s - 1], los[s]);
+        if constexpr (TWO) lds_transpose<S, R>(xb, lds, tid, los[s - 1], los[s]);
       }
       const int lo = los[s];
       const int nnew = nnew_of(los, s, R);
+      const bool last_stage = (s == ns - 1);
       static_for<R>([&](auto kc) __attribute__((always_inline)) {
         constexpr int kk = decltype(kc)::value;
         if (kk < nnew) {
@@ -441,18 +528,26 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, double2
           int lb = __builtin_amdgcn_readfirstlane(lo + kk);
           asm volatile("" : "+s"(lb));
           const int ty = d->gtype[lb];
-          if (TWO && (flags & F_REDUCE) && d->own[lb] && d->row_base >= 0) {
-            const double v = wave_sum(red_bit<R, kk>(xb, xa, ty), width);
-            if ((tid & 63) == 0) wacc[(tid >> 6) * (K + 1) + lb] += v;
+          if constexpr (TWO) {
+            if ((flags & F_REDUCE) && d->own[lb] && d->row_base >= 0) {
+              const double v = wave_sum(red_bit<R, kk>(xb, xa, ty), width);
+              if ((tid & 63) == 0) wacc[(tid >> 6) * (K + 1) + lb] += v;
+            }
           }
-          if (flags & F_ROT_A) gate_bit<R, kk>(xa, ty, d->gc[lb], d->gs[lb]);
-          if (TWO && (flags & F_ROT_B)) gate_bit<R, kk>(xb, ty, d->gc[lb], d->gs[lb]);
+          // A's gates only matter for later reductions (the remaining bits of
+          // this stage and all later stages) or its store
+          const bool need_a =
+              !TWO || (flags & F_STORE_A) || (flags & F_DRED_POST) || !last_stage || kk < nnew - 1;
+          if ((flags & F_ROT_A) && need_a) gate_bit<R, kk>(xa, ty, d->gc[lb], d->gs[lb]);
+          if constexpr (TWO) {
+            if (flags & F_ROT_B) gate_bit<R, kk>(xb, ty, d->gc[lb], d->gs[lb]);
+          }
         }
       });
     }
     if (ns > 1) {
-      lds_transpose<R>(xa, lds, tid, los[ns - 1], t);
-      if (TWO) lds_transpose<R>(xb, lds, tid, los[ns - 1], t);
+      lds_transpose<S, R>(xa, lds, tid, los[ns - 1], t);
+      if constexpr (TWO) lds_transpose<S, R>(xb, lds, tid, los[ns - 1], t);
     }
 
     // ---- diagonal ops in the natural layout, after the gates --------------
@@ -463,17 +558,22 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, double2
 #pragma unroll
       for (int a = 0; a < A; ++a) {
         const double D = diag_value<R>(dp, a);
-        if (TWO && (flags & F_DRED_POST)) xacc += 2.0 * D * (xb[a].re * xa[a].im - xb[a].im * xa[a].re);
+        if constexpr (TWO) {
+          if (flags & F_DRED_POST) xacc += 2.0 * D * imcj(xb[a], xa[a]);
+        }
         if constexpr (PH != 0) {
           if (flags & F_PHASE_POST) {
             const Cd ph = phase_of<PH>(d, D);
             xa[a] = cmul(ph, xa[a]);
-            if (TWO) xb[a] = cmul(ph, xb[a]);
+            if constexpr (TWO) xb[a] = cmul(ph, xb[a]);
           }
         }
-        if (TWO && (flags & F_OBS_DIAG)) {
-          xb[a] = {D * xa[a].re, D * xa[a].im};
-          xacc += D * (xa[a].re * xa[a].re + xa[a].im * xa[a].im);
+        if constexpr (TWO) {
+          if (flags & F_OBS_DIAG) {
+            const double ar = (double)xa[a].re, ai = (double)xa[a].im;
+            xb[a] = {D * ar, D * ai};
+            xacc += D * (ar * ar + ai * ai);
+          }
         }
       }
       if (reduce && (flags & (F_DRED_POST | F_OBS_DIAG))) {
@@ -489,8 +589,8 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, double2
         for (int k = 0; k < R; ++k)
           if (a & (1 << k)) x |= rbit[k];
         const double sg = cz_sign(x, n);
-        xa[a] = {sg * xa[a].re, sg * xa[a].im};
-        if (TWO) xb[a] = {sg * xb[a].re, sg * xb[a].im};
+        xa[a] = {(TAr)sg * xa[a].re, (TAr)sg * xa[a].im};
+        if constexpr (TWO) xb[a] = {sg * xb[a].re, sg * xb[a].im};
       }
     }
 
@@ -519,11 +619,19 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, double2
     }
     if (flags & F_STORE_A) {
 #pragma unroll
-      for (int a = 0; a < A; ++a) Stor<S>::st(d->a_out, adr[a], xa[a]);
+      for (int a = 0; a < A; ++a) {
+        if (BIG) Stor<S>::st_big(d->a_out, adr[a], xa[a]);
+        else Stor<S>::st(d->a_out, adr[a], xa[a]);
+      }
     }
-    if (TWO && (flags & F_STORE_B)) {
+    if constexpr (TWO) {
+      if (flags & F_STORE_B) {
 #pragma unroll
-      for (int a = 0; a < A; ++a) Stor<S>::st(d->b_out, adr[a], xb[a]);
+        for (int a = 0; a < A; ++a) {
+          if (BIG) Stor<S>::st_big(d->b_out, adr[a], xb[a]);
+          else Stor<S>::st(d->b_out, adr[a], xb[a]);
+        }
+      }
     }
   }
 
@@ -546,97 +654,39 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, double2
   }
 }
 
-template <class S, int R, bool TWO, int PH>
+// blockIdx.y selects one of gridDim.y consecutive descriptors: a batched
+// launch runs the same pass of gridDim.y independent circuits.
+template <class S, int R, bool TWO, int PH, class TA, bool BIG>
 __global__ __launch_bounds__(512) void k_pass(const PassDesc* __restrict__ d) {
-  extern __shared__ double2 lds_dyn[];
-  run_pass<S, R, TWO, PH>(d, lds_dyn, threadIdx.x, blockIdx.x, gridDim.x);
+  extern __shared__ __align__(16) char lds_dyn[];
+  run_pass<S, R, TWO, PH, TA, BIG>(d + blockIdx.y, lds_dyn, threadIdx.x, blockIdx.x, gridDim.x);
 }
 
-// out[i] = scale[i] * sum_{r in rows[ptr[i]..ptr[i+1])} sum_{w < cnt[r]} part[r*stride + w]
-__global__ void k_finalize(int nout, const int* __restrict__ ptr, const int* __restrict__ rows,
-                           const int* __restrict__ cnt, const double* __restrict__ scale,
-                           const double* __restrict__ part, int stride, double* __restrict__ out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nout) return;
-  double s = 0.0;
-  for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
-    const int r = rows[k];
-    double rs = 0.0;
-    for (int w = 0; w < cnt[r]; ++w) rs += part[(size_t)r * stride + w];
-    s += rs;
-  }
-  out[i] = scale[i] * s;
+// LDS bytes of a pass: the tile in the storage type plus the per-wave slots.
+__host__ __device__ inline size_t pass_lds_bytes(int dtype, int K, int threads) {
+  return (size_t)(1u << K) * (dtype == 1 ? 8 : 16) + (size_t)((threads + 63) / 64) * (kMaxK + 1) * sizeof(double);
 }
 
-}  // namespace qg
-
-// ---------------------------------------------------------------------------
-// launch wrappers used by the host planner
-// ---------------------------------------------------------------------------
-namespace qg {
-
-// per-wave accumulator slots appended to the tile buffer: [wave][kMaxK + 1]
-static inline size_t lds_acc_bytes(int threads) { return (size_t)((threads + 63) / 64) * (kMaxK + 1) * sizeof(double); }
-
-template <class S, int R, bool TWO, int PH>
-static hipError_t launch_pass_t(const PassDesc* d, int grid, int threads, hipStream_t st) {
-  const size_t lds = (size_t)threads * (1u << R) * sizeof(double2) + lds_acc_bytes(threads);
-  hipLaunchKernelGGL((k_pass<S, R, TWO, PH>), dim3(grid), dim3(threads), lds, st, d);
+// BIG: the state is 4 GiB or larger, so byte offsets need 64-bit addresses.
+template <class S, int R, bool TWO, int PH, class TA>
+hipError_t launch_pass_t(const PassDesc* d, int K, int grid, int batch, int threads, bool big, hipStream_t st) {
+  const size_t lds = pass_lds_bytes(sizeof(S) == 4 ? 1 : 0, K, threads);
+  if (big)
+    hipLaunchKernelGGL((k_pass<S, R, TWO, PH, TA, true>), dim3(grid, batch), dim3(threads), lds, st, d);
+  else
+    hipLaunchKernelGGL((k_pass<S, R, TWO, PH, TA, false>), dim3(grid, batch), dim3(threads), lds, st, d);
   return hipGetLastError();
 }
 
-template <class S, int R, bool TWO, int PH>
-static hipError_t set_attr_t() {
-  const int maxlds = (1 << kMaxK) * (int)sizeof(double2) + (int)lds_acc_bytes(1 << (kMaxK - kRegBits));
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pass<S, R, TWO, PH>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, maxlds);
-}
-
-hipError_t set_pass_attributes() {
-  hipError_t e = hipSuccess;
-#define QG_ATTR(S, PH)                                         \
-  if (e == hipSuccess) e = set_attr_t<S, 4, false, PH>();      \
-  if (e == hipSuccess) e = set_attr_t<S, 4, true, PH>();
-  QG_ATTR(float, 0) QG_ATTR(float, 1) QG_ATTR(float, 2) QG_ATTR(double, 0) QG_ATTR(double, 1) QG_ATTR(double, 2)
-#undef QG_ATTR
+template <class S, int R, bool TWO, int PH, class TA>
+hipError_t set_attr_t() {
+  const int maxlds = (int)pass_lds_bytes(sizeof(S) == 4 ? 1 : 0, kMaxK, 512);
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pass<S, R, TWO, PH, TA, false>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, maxlds);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pass<S, R, TWO, PH, TA, true>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, maxlds);
   return e;
-}
-
-// dtype: 0 complex128, 1 complex64.  reg_bits = R (4 normally; K for K < 4).
-// phase: 0 none, 1 table, 2 sincos.
-template <class S, int R>
-static hipError_t launch_r(bool two, int phase, const PassDesc* d, int grid, int threads, hipStream_t st) {
-  if (phase == 0) return two ? launch_pass_t<S, R, true, 0>(d, grid, threads, st) : launch_pass_t<S, R, false, 0>(d, grid, threads, st);
-  if (phase == 1) return two ? launch_pass_t<S, R, true, 1>(d, grid, threads, st) : launch_pass_t<S, R, false, 1>(d, grid, threads, st);
-  return two ? launch_pass_t<S, R, true, 2>(d, grid, threads, st) : launch_pass_t<S, R, false, 2>(d, grid, threads, st);
-}
-
-hipError_t launch_pass(int dtype, int reg_bits, bool two, int phase, const PassDesc* d, int grid, int threads,
-                       hipStream_t st) {
-  if (dtype == 1) {
-    switch (reg_bits) {
-      case 4: return launch_r<float, 4>(two, phase, d, grid, threads, st);
-      case 3: return launch_r<float, 3>(two, phase, d, grid, threads, st);
-      case 2: return launch_r<float, 2>(two, phase, d, grid, threads, st);
-      case 1: return launch_r<float, 1>(two, phase, d, grid, threads, st);
-    }
-  } else {
-    switch (reg_bits) {
-      case 4: return launch_r<double, 4>(two, phase, d, grid, threads, st);
-      case 3: return launch_r<double, 3>(two, phase, d, grid, threads, st);
-      case 2: return launch_r<double, 2>(two, phase, d, grid, threads, st);
-      case 1: return launch_r<double, 1>(two, phase, d, grid, threads, st);
-    }
-  }
-  return hipErrorInvalidValue;
-}
-
-hipError_t launch_finalize(int nout, const int* ptr, const int* rows, const int* cnt, const double* scale,
-                           const double* part, int stride, double* out, hipStream_t st) {
-  const int threads = 64;
-  hipLaunchKernelGGL(k_finalize, dim3((nout + threads - 1) / threads), dim3(threads), 0, st, nout, ptr, rows,
-                     cnt, scale, part, stride, out);
-  return hipGetLastError();
 }
 
 }  // namespace qg

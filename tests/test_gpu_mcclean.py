@@ -108,3 +108,27 @@ def test_invalid_axis_raises(gpu):
     c = McClean(4, obs, 2, axes=axes, angles=angles)
     with pytest.raises(ValueError):
         c.grad_run()
+
+
+@pytest.mark.parametrize('n,L,k', [(5, 6, 1), (10, 7, 3), (15, 5, 2), (18, 6, 4), (13, 4, 4)])
+@pytest.mark.parametrize('entangler', ['cnot', 'cz'])
+def test_checkpoint_uncompute_mode_c128(gpu, n, L, k, entangler):
+    """psi uncomputed in the adjoint sweep, restarted from checkpoints every k layers."""
+    from qradient_amd.circuit_logic import McClean
+    axes, angles, obs = _case(n, L, 60 + n)
+    c = McClean(n, obs, L, axes=axes, angles=angles, entangler=entangler, ckpt_every=k)
+    E, g = c.grad_run()
+    Eo, go = sv.mcclean_grad_run_adjoint(n, axes, angles, obs, entangler=entangler)
+    assert abs(E - Eo) <= C128_TOL * max(1.0, abs(Eo))
+    assert rel_err(g, go) <= C128_TOL
+
+
+@pytest.mark.parametrize('n,L,k', [(12, 12, 4), (17, 8, 3)])
+def test_checkpoint_uncompute_mode_c64(gpu, n, L, k):
+    from qradient_amd.circuit_logic import McClean
+    axes, angles, obs = _case(n, L, 70 + n)
+    c = McClean(n, obs, L, axes=axes, angles=angles, dtype='complex64', ckpt_every=k)
+    E, g = c.grad_run()
+    Eo, go = sv.mcclean_grad_run_adjoint(n, axes, angles, obs)
+    assert abs(E - Eo) <= C64_TOL * max(1.0, abs(Eo))
+    assert rel_err(g, go) <= C64_TOL

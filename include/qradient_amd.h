@@ -129,6 +129,9 @@ int qg_plan_pass_times(const qg_plan* plan, int cap, float* ms, int* kinds, int*
  * complex64 forward and history-adjoint passes.  Used by the autotuner;
  * rejects shapes that change the number of passes per layer. */
 int qg_plan_tune(qg_plan* plan, int kmax, int cmin, int r_fwd, int r_rev);
+/* on = 1 (default): circuits with n <= kmax run whole in one workgroup each
+ * (one launch per call, FP64 throughout); on = 0 uses the pass kernels. */
+int qg_plan_set_fused(qg_plan* plan, int on);
 int qg_plan_destroy(qg_plan* plan);
 
 /* ---- device-resident State (qradient/physical_components/state.py) --------

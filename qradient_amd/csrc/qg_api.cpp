@@ -32,6 +32,27 @@ hipError_t simple_scale(int dtype, void* v, int n, double re, double im, hipStre
 hipError_t simple_convert_in(int dtype, const void* in_c128, void* out, int n, hipStream_t st);
 hipError_t simple_convert_out(int dtype, const void* in, void* out_c128, int n, hipStream_t st);
 constexpr int kDotBlocks = 512;
+hipError_t launch_fused(const void* descs, int batch, int n, hipStream_t st);
+hipError_t set_fused_attributes();
+size_t fused_desc_size();
+
+// Mirror of the device-side FusedDesc in qg_fused.hip (same layout).
+struct FusedDescH {
+  int32_t n, L, kind, ent;
+  const int32_t* gtype;
+  const double* gcs;
+  const double* params;
+  const DTerm* dterms;
+  int32_t ndterms;
+  const DTerm* xy;
+  int32_t nxy;
+  const uint32_t* map;
+  const double* init_tab;
+  double* out;
+  int32_t grad;
+  int32_t psi_c64;
+  void* psi_out;
+};
 }  // namespace qg
 
 using namespace qg;
@@ -101,7 +122,8 @@ extern "C" int qg_ctx_create(int device, void* stream, qg_ctx** out) {
     c->own_stream = true;
   }
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
-      set_pass_attributes() != hipSuccess) {
+      set_pass_attributes() != hipSuccess || set_fused_attributes() != hipSuccess ||
+      fused_desc_size() != sizeof(FusedDescH)) {
     delete c;
     return fail(QG_ERR_HIP, "qg_ctx_create: event creation or kernel attribute setup failed");
   }
@@ -266,6 +288,11 @@ struct qg_plan {
   int kmax = kMaxK, cmin = 2;
   int r_fwd = 5, r_rev = 5;  // amplitudes per thread (log2) of forward / history-adjoint passes (complex64)
   int chunk = 1;  // circuits resident at once (QAOA batches)
+  bool fused = true;       // n <= kmax: whole circuit per workgroup (qg_fused.hip)
+  char* h_arena = nullptr;  // per-call tables, pinned host
+  char* d_arena = nullptr;  // and their device copy
+  size_t arena_cap = 0;
+  void* d_fstate = nullptr;  // final state of the last fused McClean run
   std::vector<Geo> geo;
   size_t S = 0;  // bytes per state
   // workspace
@@ -504,6 +531,13 @@ static void free_small(qg_plan* p) {
   hipFree(p->d_desc);
   if (p->h_desc) hipHostFree(p->h_desc);
   if (p->h_out) hipHostFree(p->h_out);
+  if (p->h_arena) hipHostFree(p->h_arena);
+  hipFree(p->d_arena);
+  hipFree(p->d_fstate);
+  p->h_arena = nullptr;
+  p->d_arena = nullptr;
+  p->d_fstate = nullptr;
+  p->arena_cap = 0;
   p->d_map = nullptr;
   p->d_init = nullptr;
   p->d_terms = p->d_xy = nullptr;
@@ -780,6 +814,96 @@ static int upload_finalize(qg_plan* p, const std::vector<std::vector<int>>& out_
   return QG_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// per-call arena: tables built in pinned host memory, copied with one H2D
+// ---------------------------------------------------------------------------
+struct Arena {
+  qg_plan* p;
+  size_t used = 0;
+  // returns the host pointer; *dev receives the device address of the same bytes
+  template <class T>
+  T* take(size_t count, T** dev) {
+    used = (used + 255) & ~(size_t)255;
+    T* h = (T*)(p->h_arena + used);
+    *dev = (T*)(p->d_arena + used);
+    used += sizeof(T) * count;
+    return h;
+  }
+};
+
+static int ensure_arena(qg_plan* p, size_t bytes) {
+  if (bytes <= p->arena_cap) return QG_OK;
+  HIPCHK(hipStreamSynchronize(p->ctx->stream));
+  if (p->h_arena) hipHostFree(p->h_arena);
+  hipFree(p->d_arena);
+  p->h_arena = nullptr;
+  p->d_arena = nullptr;
+  p->arena_cap = 0;
+  HIPCHK(hipHostMalloc(&p->h_arena, bytes, hipHostMallocDefault));
+  HIPCHK(hipMalloc(&p->d_arena, bytes));
+  p->arena_cap = bytes;
+  return QG_OK;
+}
+
+// Runs B circuits through the whole-circuit kernel.  The tables of circuit b
+// are written by fill(b, desc, arena); results land in out[b * per_out ...].
+template <class Fill>
+static int run_fused(qg_plan* p, int B, size_t bytes_per_circuit, int per_out, bool grad, void* psi_out, Fill fill,
+                     double* out) {
+  qg_ctx* c = p->ctx;
+  const size_t need = (size_t)B * (bytes_per_circuit + 16 * 256 + sizeof(FusedDescH) + sizeof(double) * per_out);
+  CHK(ensure_arena(p, need));
+  Arena ar{p};
+  FusedDescH* d_desc = nullptr;
+  FusedDescH* h_desc = ar.take<FusedDescH>(B, &d_desc);
+  double* d_out = nullptr;
+  ar.take<double>((size_t)B * per_out, &d_out);
+  for (int b = 0; b < B; ++b) {
+    std::memset(&h_desc[b], 0, sizeof(FusedDescH));
+    h_desc[b].n = p->n;
+    h_desc[b].L = p->L;
+    h_desc[b].ent = p->ent;
+    h_desc[b].map = p->d_map;
+    h_desc[b].init_tab = p->d_init;
+    h_desc[b].out = d_out + (size_t)b * per_out;
+    h_desc[b].grad = grad ? 1 : 0;
+    h_desc[b].psi_c64 = p->dtype == QG_C64 ? 1 : 0;
+    h_desc[b].psi_out = psi_out;
+    CHK(fill(b, h_desc[b], ar));
+  }
+  HIPCHK(hipMemcpyAsync(p->d_arena, p->h_arena, ar.used, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  HIPCHK(launch_fused(d_desc, B, p->n, c->stream));
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  const int nout = grad ? per_out : 1;
+  for (int b = 0; b < B && !grad; ++b)
+    HIPCHK(hipMemcpyAsync(out + (size_t)b * per_out, d_out + (size_t)b * per_out, sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+  if (grad)
+    HIPCHK(hipMemcpyAsync(out, d_out, sizeof(double) * (size_t)B * nout, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipEventElapsedTime(&p->last_ms, c->ev0, c->ev1));
+  p->launches.clear();
+  p->pass_ms.clear();
+  p->pass_kind.clear();
+  return QG_OK;
+}
+
+static int obs_tables(const ObsSplit& obs, FusedDescH& d, Arena& ar) {
+  DTerm* dd = nullptr;
+  DTerm* hd = ar.take<DTerm>(std::max<size_t>(1, obs.diag.size()), &dd);
+  std::copy(obs.diag.begin(), obs.diag.end(), hd);
+  DTerm* dx = nullptr;
+  DTerm* hx = ar.take<DTerm>(std::max<size_t>(1, obs.xy.size()), &dx);
+  std::copy(obs.xy.begin(), obs.xy.end(), hx);
+  d.dterms = dd;
+  d.ndterms = (int)obs.diag.size();
+  d.xy = dx;
+  d.nxy = (int)obs.xy.size();
+  return QG_OK;
+}
+
 static int mcclean_run(qg_plan* p, const int32_t* axes, const double* angles, const qg_term* terms, int nterms,
                        const double* ini_state, double* E_out, double* grad_out, bool grad) {
   if (!p || p->kind != PK_MCCLEAN) return fail(QG_ERR_INVALID, "not a McClean plan");
@@ -793,10 +917,36 @@ static int mcclean_run(qg_plan* p, const int32_t* axes, const double* angles, co
   CHK(split_observable(n, terms, nterms, obs));
   if ((int)obs.diag.size() > p->max_terms || (int)obs.xy.size() > p->max_terms)
     return fail(QG_ERR_INVALID, "too many observable terms");
-  if (!obs.xy.empty())
-    return fail(QG_ERR_UNSUPPORTED, "observables with x/y terms are not supported by grad_run in this build");
   qg_ctx* c = p->ctx;
   HIPCHK(hipSetDevice(c->device));
+  if (p->fused && n <= p->kmax && !ini_state) {
+    // the whole circuit in one workgroup (qg_fused.hip): gates as (type, cos, sin)
+    std::vector<double> res(1 + (size_t)L * n);
+    auto fill = [&](int, FusedDescH& d, Arena& ar) -> int {
+      d.kind = 0;
+      int32_t* dt = nullptr;
+      int32_t* ht = ar.take<int32_t>((size_t)L * n, &dt);
+      double* dg = nullptr;
+      double* hg = ar.take<double>((size_t)2 * L * n, &dg);
+      for (int k = 0; k < L * n; ++k) {
+        ht[k] = axes[k] == 0 ? G_X : axes[k] == 1 ? G_Y : G_Z;
+        hg[2 * k] = std::cos(0.5 * angles[k]);
+        hg[2 * k + 1] = std::sin(0.5 * angles[k]);
+      }
+      d.gtype = dt;
+      d.gcs = dg;
+      return obs_tables(obs, d, ar);
+    };
+    const size_t bytes = (size_t)L * n * 20 + (obs.diag.size() + obs.xy.size() + 2) * sizeof(DTerm);
+    if (!p->d_fstate) HIPCHK(hipMalloc(&p->d_fstate, ((size_t)16) << n));
+    CHK(run_fused(p, 1, bytes, 1 + L * n, grad, p->d_fstate, fill, res.data()));
+    *E_out = res[0];
+    if (grad) std::memcpy(grad_out, res.data() + 1, sizeof(double) * L * n);
+    p->last_state_ptr = p->d_fstate;
+    return QG_OK;
+  }
+  if (!obs.xy.empty())
+    return fail(QG_ERR_UNSUPPORTED, "observables with x/y terms need n <= 13 (fused path) in this build");
   CHK(ensure_workspace(p));
   if (!obs.diag.empty())
     HIPCHK(hipMemcpyAsync(p->d_terms, obs.diag.data(), sizeof(DTerm) * obs.diag.size(), hipMemcpyHostToDevice,
@@ -955,6 +1105,32 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
   const int Wi = (int)W;
   qg_ctx* c = p->ctx;
   HIPCHK(hipSetDevice(c->device));
+  if (p->fused && n <= p->kmax) {
+    const int per = 1 + 2 * L;
+    std::vector<double> res((size_t)B * per);
+    auto fill = [&](int b, FusedDescH& d, Arena& ar) -> int {
+      d.kind = 1;
+      double* dp = nullptr;
+      double* hp = ar.take<double>((size_t)2 * L, &dp);
+      for (int i = 0; i < L; ++i) {
+        hp[2 * i] = betas[(size_t)b * L + i];
+        hp[2 * i + 1] = gammas[(size_t)b * L + i];
+      }
+      d.params = dp;
+      return obs_tables(obs[b], d, ar);
+    };
+    size_t mx = 0;
+    for (int b = 0; b < B; ++b) mx = std::max(mx, obs[b].diag.size());
+    const size_t bytes = (size_t)L * 16 + (mx + 2) * sizeof(DTerm);
+    CHK(run_fused(p, B, bytes, per, grad, nullptr, fill, res.data()));
+    for (int b = 0; b < B; ++b) {
+      E_out[b] = res[(size_t)b * per];
+      if (grad)
+        for (int k = 0; k < 2 * L; ++k) grad_out[(size_t)b * 2 * L + k] = res[(size_t)b * per + 1 + k];
+    }
+    p->last_state_ptr = nullptr;
+    return QG_OK;
+  }
   CHK(ensure_workspace(p));
   const int chunk = p->chunk;
   const int lut_len = 2 * Wi + 1;
@@ -1400,5 +1576,11 @@ extern "C" int qg_plan_tune(qg_plan* p, int kmax, int cmin, int r_fwd, int r_rev
     p->ws_owned = false;
   }
   p->last_state_ptr = nullptr;
+  return QG_OK;
+}
+
+extern "C" int qg_plan_set_fused(qg_plan* p, int on) {
+  if (!p) return fail(QG_ERR_INVALID, "NULL plan");
+  p->fused = on != 0;
   return QG_OK;
 }

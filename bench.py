@@ -101,12 +101,119 @@ def cpu_baseline(axes, angles, obs):
                                                                                         N_LAYERS // layers)}
 
 
+def _line(metric, value, unit, steps, warmup, ms, dtype, config, **extra):
+    out = {'metric': metric, 'value': value, 'unit': unit, 'n_gpus': 1, 'steps': steps, 'warmup': warmup,
+           'ms_per_step': ms, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': dtype,
+           'data': 'synthetic (seeded)', 'config': config}
+    out.update(extra)
+    return out
+
+
+def bench_qaoa(args, sync):
+    """512 seeded random 3-regular MaxCut graphs, 20 qubits, p = 8: one batched grad_run per step."""
+    from qradient_amd.batch import QaoaBatch
+    from qradient_amd.optimization_problems import MaxCut
+    n, p, B = 20, 8, 512
+    rs = np.random.RandomState(SEED)
+    obs = [MaxCut.regular3(n, rs).to_observable() for _ in range(B)]
+    gammas = np.pi * rs.rand(B, p)
+    betas = np.pi * rs.rand(B, p)
+    res = {}
+    for dtype in ('complex64', 'complex128'):
+        qb = QaoaBatch(n, p, B, dtype=dtype)
+        packed = qb.pack(obs)
+        steps = args.steps if dtype == 'complex64' else max(2, args.steps // 2)
+        for _ in range(args.warmup):
+            qb.grad_run(packed, betas, gammas)
+        sync()
+        t0 = time.perf_counter()
+        dev = 0.0
+        for _ in range(steps):
+            E, g = qb.grad_run(packed, betas, gammas)
+            dev += qb.last_device_ms()
+        sync()
+        dt = time.perf_counter() - t0
+        res[dtype] = dict(ms=dt / steps * 1e3, dev_ms=dev / steps, E=E, g=g, steps=steps)
+        del qb
+    c64, c128 = res['complex64'], res['complex128']
+    rel = np.linalg.norm((c64['g'] - c128['g']).reshape(B, -1), axis=1) / np.linalg.norm(
+        c128['g'].reshape(B, -1), axis=1)
+    sbytes = (2 ** n) * 8
+    b_alg = (6 * p + 3) * sbytes * B
+    return _line('QAOA grad_run circuits/s (512 x MaxCut 3-regular, 20 qubits, p=8, complex64 state + FP64 '
+                 'reductions)', B * 1e3 / c64['ms'], 'circuits/s', c64['steps'], args.warmup, c64['ms'],
+                 'c64 (FP64 arithmetic and reductions)',
+                 {'workload': 'qaoa_maxcut_reg3_n20_p8_batch512', 'qubits': n, 'layers': p, 'batch': B,
+                  'seed': SEED},
+                 device_ms_per_step=c64['dev_ms'],
+                 c128_plain={'circuits_per_s': B * 1e3 / c128['ms'], 'ms_per_step': c128['ms']},
+                 c64_vs_c128_grad_rel_err_max=float(rel.max()),
+                 effective_gbs=b_alg / (c64['dev_ms'] * 1e-3) / 1e9,
+                 b_alg_bytes=b_alg)
+
+
+def bench_serve(args, sync):
+    """Poisson stream of McClean requests (n ~ U{12..28}, L ~ U{4..64}) through GradServer."""
+    from qradient_amd import serve
+    nreq = max(50, 20 * args.steps)
+    rate = 20.0
+    trace = serve.poisson_trace(nreq, rate, SEED, (12, 28), (4, 64))
+    t0 = time.perf_counter()
+    st = serve.replay(trace, slots=8)
+    dt = time.perf_counter() - t0
+    return _line('serving grad_run circuits/s with p50/p99 latency (McClean n~U{12..28}, L~U{4..64}, Poisson '
+                 'arrivals at %.0f/s)' % rate, st['circuits_per_s'], 'circuits/s', nreq, 0, dt / nreq * 1e3,
+                 'c64 (FP64 arithmetic and reductions)',
+                 {'workload': 'serve_mcclean_poisson', 'requests': nreq, 'rate_per_s': rate, 'slots': 8,
+                  'seed': SEED}, latency_ms={'p50': st['p50_ms'], 'p99': st['p99_ms'], 'mean': st['mean_ms']},
+                 peak_bytes=st['peak_bytes'], budget_bytes=st['budget_bytes'])
+
+
+def bench_large(args, sync):
+    """McClean 30 qubits, 100 layers: complex64 with checkpoints every k layers vs complex128."""
+    from qradient_amd.circuit_logic import McClean
+    n, L = 30, 100
+    rs = np.random.RandomState(SEED)
+    axes = (3 * rs.rand(L, n)).astype(int)
+    angles = 2 * np.pi * rs.rand(L, n)
+    zz = np.full((n, n), None)
+    for i, w in enumerate(rs.standard_normal(n - 1)):
+        zz[i, i + 1] = float(w)
+    obs = {'zz': zz}
+    res = {}
+    for dtype, k in (('complex64', 10), ('complex128', 20)):
+        circ = McClean(n, obs, L, axes=axes, angles=angles, dtype=dtype, ckpt_every=k)
+        t0 = time.perf_counter()
+        E, g = circ.grad_run()
+        sync()
+        dt = time.perf_counter() - t0
+        res[dtype] = dict(ms=dt * 1e3, E=E, g=g, k=k)
+        circ._destroy_plan()
+    c64, c128 = res['complex64'], res['complex128']
+    rel = float(np.linalg.norm(c64['g'] - c128['g']) / np.linalg.norm(c128['g']))
+    sb = (2 ** n) * 8
+    b_alg = (6 * L + 3 + 2 * -(-L // c64['k'])) * sb
+    return _line('grad_run circuits/s (McClean 30 qubits, 100 layers, complex64 state + FP64 reductions, '
+                 'checkpoints every %d layers)' % c64['k'], 1e3 / c64['ms'], 'circuits/s', 1, 0, c64['ms'],
+                 'c64 (FP64 arithmetic and reductions)',
+                 {'workload': 'mcclean_grad_run_n30_L100_zzchain', 'qubits': n, 'layers': L, 'seed': SEED},
+                 c128_plain={'ms_per_step': c128['ms'], 'ckpt_every': c128['k']},
+                 c64_vs_c128_grad_rel_err=rel, effective_gbs=b_alg / (c64['ms'] * 1e-3) / 1e9, b_alg_bytes=b_alg)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=10)
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--workload', default='mcclean24', choices=['mcclean24', 'qaoa512', 'serve', 'mcclean30'],
+                    help='mcclean24 (default, the headline line); qaoa512: 512 batched QAOA MaxCut grad_runs '
+                         '(20 qubits, p=8); serve: Poisson request stream through the serving loop; mcclean30: '
+                         'McClean 30 qubits, 100 layers, complex64 checkpoints vs complex128')
+    ap.add_argument('--probe', action='store_true',
+                    help='short run for hardware-counter collection: two complex64 grad_runs of the workload '
+                         'and one plain 1 GiB device copy (the calibration kernel for FETCH_SIZE / WRITE_SIZE)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU: this build has no multi-GPU path (every configuration is a '
@@ -119,7 +226,24 @@ def main():
     def sync():
         torch.cuda.synchronize()
 
+    if args.workload != 'mcclean24':
+        print(json.dumps({'qaoa512': bench_qaoa, 'serve': bench_serve, 'mcclean30': bench_large}[args.workload](
+            args, sync)))
+        return
     axes, angles, obs = workload()
+    if args.probe:
+        circ = McClean(N_QUBITS, obs, N_LAYERS, axes=axes, angles=angles, dtype='complex64')
+        for _ in range(2):
+            circ.grad_run()
+        circ._destroy_plan()
+        nbytes = 1 << 30
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device='cuda')
+        dst = torch.empty_like(src)
+        sync()
+        dst.copy_(src)
+        sync()
+        print(json.dumps({'probe': 'mcclean_grad_run_n24_L40 complex64 x2, then copy', 'copy_bytes': nbytes}))
+        return
     res = {}
     grads = {}
     for dtype in ('complex64', 'complex128'):

@@ -124,11 +124,18 @@ def iptr(a):
     return a.ctypes.data_as(_ip)
 
 
+# numpy mirror of struct qg_term (24 bytes: 4 x int32, float64)
+TERM_DTYPE = np.dtype([('kind', '<i4'), ('i', '<i4'), ('j', '<i4'), ('pad', '<i4'), ('w', '<f8')])
+assert TERM_DTYPE.itemsize == ctypes.sizeof(QgTerm)
+
+
 def terms_array(terms):
     """[(kind, i, j, w)] -> (ctypes array, count)."""
     arr = (QgTerm * max(1, len(terms)))()
-    for k, (kind, i, j, w) in enumerate(terms):
-        arr[k].kind, arr[k].i, arr[k].j, arr[k].w = int(kind), int(i), int(j), float(w)
+    if terms:
+        a = np.frombuffer(arr, dtype=TERM_DTYPE)
+        t = np.asarray([(int(k), int(i), int(j), 0, float(w)) for (k, i, j, w) in terms], dtype=TERM_DTYPE)
+        a[:len(terms)] = t
     return arr, len(terms)
 
 

@@ -12,6 +12,16 @@ from . import _native
 from .physical_components import Observable
 
 
+class PackedObservables:
+    """Classical Hamiltonians of a batch in the C-ABI term layout (see QaoaBatch.pack)."""
+
+    def __init__(self, arr, off):
+        self.arr, self.off = arr, off
+
+    def __len__(self):
+        return len(self.off) - 1
+
+
 class QaoaBatch:
     """Plan for up to ``batch`` QAOA circuits of ``n`` qubits and ``p`` layers."""
 
@@ -29,7 +39,16 @@ class QaoaBatch:
             _native._lib.qg_plan_destroy(h)
             self._plan = None
 
+    def pack(self, observables):
+        """Pre-packs the classical Hamiltonians of a batch; the result can be passed to
+        grad_run / expec_val in place of the observable list (an optimiser loop over fixed
+        graphs then converts them once)."""
+        arr, off = self._terms(observables)
+        return PackedObservables(arr, off)
+
     def _terms(self, observables):
+        if isinstance(observables, PackedObservables):
+            return observables.arr, observables.off
         terms, off = [], [0]
         for o in observables:
             ob = o if isinstance(o, Observable) else Observable(o)
@@ -48,6 +67,8 @@ class QaoaBatch:
     def grad_run(self, observables, betas, gammas):
         """(E[B], grad[B, p, 2]) with grad[b, i] = (dE/dbeta_i, dE/dgamma_i) of circuit b."""
         B = len(observables)
+        if B > self.batch:
+            raise ValueError('{} circuits, the plan holds {}'.format(B, self.batch))
         arr, off = self._terms(observables)
         b, g = self._params(betas, gammas, B)
         E = np.empty(B)

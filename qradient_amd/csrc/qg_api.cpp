@@ -16,7 +16,7 @@
 namespace qg {
 hipError_t set_pass_attributes();
 hipError_t launch_pass(int dtype, int R, bool two, int phase, bool ta_float, const PassDesc* d, int K, int grid,
-                       int batch, int threads, int n, hipStream_t st);
+                       int batch, int threads, int n, hipStream_t st, int* occ);
 hipError_t launch_finalize(int nout, const int* ptr, const int* rows, const int* cnt, const double* scale,
                            const double* part, int stride, double* out, hipStream_t st);
 hipError_t simple_rot1(int dtype, void* v, int n, int b, int type, double c, double s, hipStream_t st);
@@ -286,7 +286,7 @@ struct qg_plan {
   int nckpt = 0;
   size_t off_work = 0, off_ckpt = 0;
   int kmax = kMaxK, cmin = 2;
-  int r_fwd = 5, r_rev = 5;  // amplitudes per thread (log2) of forward / history-adjoint passes (complex64)
+  int r_fwd = 4, r_rev = 4;  // amplitudes per thread (log2) of forward / history-adjoint passes (complex64)
   int chunk = 1;  // circuits resident at once (QAOA batches)
   bool fused = true;       // n <= kmax: whole circuit per workgroup (qg_fused.hip)
   char* h_arena = nullptr;  // per-call tables, pinned host
@@ -347,19 +347,43 @@ static int ensure_workspace(qg_plan* p) {
   return QG_OK;
 }
 
-static size_t lds_bytes_of(int dtype, int K, int threads) {
-  return (size_t)(1u << K) * (dtype == QG_C64 ? 8 : 16) + (size_t)((threads + 63) / 64) * (kMaxK + 1) * 8;
+static size_t lds_bytes_of(int dtype, int K, int threads) {  // the larger (adjoint) variant
+  return (size_t)(1u << K) * (dtype == QG_C64 ? 8 : 16) + (size_t)((threads + 63) / 64) * (kMaxK + 1) * 8 +
+         (dtype == QG_C64 ? (size_t)K * threads * 8 : 0);
 }
 
-static int grid_for_pass(const qg_plan* p, const Geo& g) {
-  const size_t lds = lds_bytes_of(p->dtype, g.K, g.threads);
-  int per_cu = (int)std::max<size_t>(1, (160 * 1024) / lds);
-  per_cu = std::min(per_cu, std::max(1, 512 / g.threads));  // 2 waves per SIMD at <= 256 VGPRs
+// Workgroups per pass launch: as many as are resident at once on the device
+// (occupancy of the kernel variant, by its registers and LDS), rounded down to
+// a power of two and at most one per tile.
+static int grid_for_pass(const qg_plan* p, const Geo& g, bool two, bool ta_float, int phase) {
+  int occ = 0;
+  if (launch_pass(p->dtype, g.R, two, phase, ta_float, nullptr, g.K, 0, 1, g.threads, p->n, nullptr, &occ) !=
+          hipSuccess ||
+      occ < 1) {
+    const size_t lds = lds_bytes_of(p->dtype, g.K, g.threads);
+    occ = (int)std::max<size_t>(1, (160 * 1024) / lds);
+    occ = std::min(occ, std::max(1, 512 / g.threads));
+  }
   const uint64_t ntiles = 1ull << (p->n - g.K);
-  uint64_t target = (uint64_t)p->ctx->num_cu * per_cu;
+  uint64_t target = (uint64_t)p->ctx->num_cu * occ;
   uint64_t grid = 1;
   while (grid * 2 <= target && grid * 2 <= ntiles) grid *= 2;
   return (int)grid;
+}
+
+// The largest grid any pass variant of these geometries may use (the stride
+// of the partial rows).
+static int grid_max(const qg_plan* p) {
+  int gm = 1;
+  for (const Geo& g : p->geo)
+    for (int role = 0; role < 3; ++role)
+      for (int phase = 0; phase < 3; ++phase) {
+        bool tf;
+        const Geo G = with_role(g, p->dtype, role, &tf);
+        gm = std::max(gm, grid_for_pass(p, G, role != ROLE_FWD, tf, phase));
+        if (role == ROLE_FWD) gm = std::max(gm, grid_for_pass(p, G, true, false, phase));
+      }
+  return gm;
 }
 
 static int alloc_small(qg_plan* p, int ndesc, int nrows, int nout) {
@@ -448,15 +472,26 @@ static void desc_geo(PassDesc& d, int n, const Geo& g) {
   d.row_base = -1;
   for (int k = 0; k < kMaxK; ++k) {
     d.gtype[k] = G_NONE;
-    d.gc[k] = 1.0;
-    d.gs[k] = 0.0;
+    d.gr[k] = 0.0;
+    d.gk[k] = 1.0;
   }
 }
 
+// U = c I - i s P with c = cos(angle/2), s = sign * sin(angle/2), stored in the
+// factored form the pass kernels apply (one FMA per real output): the larger
+// of |c|, |s| is the factor, so the ratio stays in [-1, 1].
 static void set_gate(PassDesc& d, int lb, int axis, double angle, double sign) {
-  d.gtype[lb] = axis == 0 ? G_X : axis == 1 ? G_Y : G_Z;
-  d.gc[lb] = std::cos(0.5 * angle);
-  d.gs[lb] = sign * std::sin(0.5 * angle);
+  const int type = axis == 0 ? G_X : axis == 1 ? G_Y : G_Z;
+  const double c = std::cos(0.5 * angle), s = sign * std::sin(0.5 * angle);
+  if (std::fabs(c) >= std::fabs(s)) {
+    d.gtype[lb] = type;
+    d.gk[lb] = c;
+    d.gr[lb] = s / c;
+  } else {
+    d.gtype[lb] = type + 4;
+    d.gk[lb] = s;
+    d.gr[lb] = c / s;
+  }
 }
 
 static int run_launches(qg_plan* p, int ndesc) {
@@ -476,7 +511,7 @@ static int run_launches(qg_plan* p, int ndesc) {
     if (p->profile) HIPCHK(hipEventRecord(p->pev[k], c->stream));
     if (l.kind == 0) {
       HIPCHK(launch_pass(p->dtype, l.R, l.two, l.phase, l.ta_float, p->d_desc + l.desc, l.K, l.grid, l.batch,
-                         l.threads, p->n, c->stream));
+                         l.threads, p->n, c->stream, nullptr));
     }
   }
   if (p->profile) HIPCHK(hipEventRecord(p->pev[nl], c->stream));
@@ -592,12 +627,7 @@ static int mcclean_layout(qg_plan* p) {
   p->off_ini = p->off_lam + 2 * p->S;
   p->off_prep = p->off_ini + p->S;
   p->ws_bytes = p->off_prep + p->S;
-  p->gmax = 1;
-  for (const Geo& g : p->geo)
-    for (int role = 0; role < 3; ++role) {
-      bool tf;
-      p->gmax = std::max(p->gmax, grid_for_pass(p, with_role(g, p->dtype, role, &tf)));
-    }
+  p->gmax = grid_max(p);
   const int npass = 2 * P * L + 2 * P;  // forward + reverse + ini-state preparation
   p->nrows = (P * L + 1) * (kMaxK + 1);
   CHK(alloc_small(p, npass, p->nrows, 1 + L * n));
@@ -670,7 +700,7 @@ static int mcclean_schedule(qg_plan* p, const int32_t* axes, const double* angle
     l.K = g.K;
     l.ta_float = tf;
     l.threads = g.threads;
-    l.grid = grid_for_pass(p, g);
+    l.grid = grid_for_pass(p, g, two, tf, 0);
     l.phase = 0;
     l.two = two;
     l.kind = 0;
@@ -1048,11 +1078,7 @@ extern "C" int qg_qaoa_plan(qg_ctx* ctx, int n, int p_layers, int dtype, int bat
   p->off_hist = 0;
   p->off_lam = (size_t)p->chunk * P * p_layers * p->S;
   p->ws_bytes = p->off_lam + (size_t)p->chunk * 2 * p->S;
-  for (const Geo& g : p->geo)
-    for (int role = 0; role < 3; ++role) {
-      bool tf;
-      p->gmax = std::max(p->gmax, grid_for_pass(p, with_role(g, dtype, role, &tf)));
-    }
+  p->gmax = grid_max(p);
   const int ndesc = p->chunk * 2 * P * p_layers;
   p->nrows = p->chunk * (P * p_layers + 1) * (kMaxK + 1);
   int r = alloc_small(p, ndesc, p->nrows, p->chunk * (1 + 2 * p_layers));
@@ -1196,7 +1222,7 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
       l.K = G.K;
       l.ta_float = tf;
       l.threads = G.threads;
-      l.grid = grid_for_pass(p, G);
+      l.grid = grid_for_pass(p, G, two, tf, phase);
       l.phase = phase;
       l.two = two;
       l.kind = 0;

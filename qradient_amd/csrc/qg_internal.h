@@ -53,9 +53,12 @@ struct PassDesc {
   void* a_out;
   const void* b_in;
   void* b_out;
-  // gates, indexed by local bit: U = [[c, ...]] with the sign of s chosen by the planner
-  double gc[kMaxK];
-  double gs[kMaxK];
+  // gates, indexed by local bit, in factored form (inverses have s negated):
+  //   U = c I - i s P = gk * (I - i gr P)   (form 0, |c| >= |s|: gk = c, gr = s / c)
+  //                   = gk * (gr I - i P)   (form 1, |s| >  |c|: gk = s, gr = c / s)
+  // gtype = G_X / G_Y / G_Z plus 4 * form; G_NONE has gk = 1.
+  double gr[kMaxK];
+  double gk[kMaxK];
   int32_t gtype[kMaxK];
   int32_t own[kMaxK];    // 1: accumulate the reduction of this local bit into row_base + bit
   const uint32_t* map;   // ladder map columns (n entries), or null

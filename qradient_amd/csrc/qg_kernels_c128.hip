@@ -6,24 +6,24 @@ namespace qg {
 
 template <int R>
 static hipError_t launch_c128_r(bool two, int phase, const PassDesc* d, int K, int grid, int batch, int threads,
-                                bool big, hipStream_t st) {
+                                bool big, hipStream_t st, int* occ) {
   if (!two) {
-    if (phase == 0) return launch_pass_t<double, R, false, 0, double>(d, K, grid, batch, threads, big, st);
-    if (phase == 1) return launch_pass_t<double, R, false, 1, double>(d, K, grid, batch, threads, big, st);
-    return launch_pass_t<double, R, false, 2, double>(d, K, grid, batch, threads, big, st);
+    if (phase == 0) return launch_pass_t<double, R, false, 0, double>(d, K, grid, batch, threads, big, st, occ);
+    if (phase == 1) return launch_pass_t<double, R, false, 1, double>(d, K, grid, batch, threads, big, st, occ);
+    return launch_pass_t<double, R, false, 2, double>(d, K, grid, batch, threads, big, st, occ);
   }
-  if (phase == 0) return launch_pass_t<double, R, true, 0, double>(d, K, grid, batch, threads, big, st);
-  if (phase == 1) return launch_pass_t<double, R, true, 1, double>(d, K, grid, batch, threads, big, st);
-  return launch_pass_t<double, R, true, 2, double>(d, K, grid, batch, threads, big, st);
+  if (phase == 0) return launch_pass_t<double, R, true, 0, double>(d, K, grid, batch, threads, big, st, occ);
+  if (phase == 1) return launch_pass_t<double, R, true, 1, double>(d, K, grid, batch, threads, big, st, occ);
+  return launch_pass_t<double, R, true, 2, double>(d, K, grid, batch, threads, big, st, occ);
 }
 
 hipError_t launch_pass_c128(int R, bool two, int phase, const PassDesc* d, int K, int grid, int batch, int threads,
-                            bool big, hipStream_t st) {
+                            bool big, hipStream_t st, int* occ) {
   switch (R) {
-    case 4: return launch_c128_r<4>(two, phase, d, K, grid, batch, threads, big, st);
-    case 3: return launch_c128_r<3>(two, phase, d, K, grid, batch, threads, big, st);
-    case 2: return launch_c128_r<2>(two, phase, d, K, grid, batch, threads, big, st);
-    case 1: return launch_c128_r<1>(two, phase, d, K, grid, batch, threads, big, st);
+    case 4: return launch_c128_r<4>(two, phase, d, K, grid, batch, threads, big, st, occ);
+    case 3: return launch_c128_r<3>(two, phase, d, K, grid, batch, threads, big, st, occ);
+    case 2: return launch_c128_r<2>(two, phase, d, K, grid, batch, threads, big, st, occ);
+    case 1: return launch_c128_r<1>(two, phase, d, K, grid, batch, threads, big, st, occ);
   }
   return hipErrorInvalidValue;
 }

@@ -6,42 +6,42 @@ namespace qg {
 
 template <int R>
 static hipError_t launch_c64_r(bool two, int phase, bool ta_float, const PassDesc* d, int K, int grid, int batch,
-                               int threads, bool big, hipStream_t st) {
+                               int threads, bool big, hipStream_t st, int* occ) {
   if (!two) {
-    if (phase == 0) return launch_pass_t<float, R, false, 0, double>(d, K, grid, batch, threads, big, st);
-    if (phase == 1) return launch_pass_t<float, R, false, 1, double>(d, K, grid, batch, threads, big, st);
-    return launch_pass_t<float, R, false, 2, double>(d, K, grid, batch, threads, big, st);
+    if (phase == 0) return launch_pass_t<float, R, false, 0, double>(d, K, grid, batch, threads, big, st, occ);
+    if (phase == 1) return launch_pass_t<float, R, false, 1, double>(d, K, grid, batch, threads, big, st, occ);
+    return launch_pass_t<float, R, false, 2, double>(d, K, grid, batch, threads, big, st, occ);
   }
   if constexpr (R == 5) {
     if (ta_float) {
-      if (phase == 0) return launch_pass_t<float, R, true, 0, float>(d, K, grid, batch, threads, big, st);
-      if (phase == 1) return launch_pass_t<float, R, true, 1, float>(d, K, grid, batch, threads, big, st);
-      return launch_pass_t<float, R, true, 2, float>(d, K, grid, batch, threads, big, st);
+      if (phase == 0) return launch_pass_t<float, R, true, 0, float>(d, K, grid, batch, threads, big, st, occ);
+      if (phase == 1) return launch_pass_t<float, R, true, 1, float>(d, K, grid, batch, threads, big, st, occ);
+      return launch_pass_t<float, R, true, 2, float>(d, K, grid, batch, threads, big, st, occ);
     }
     return hipErrorInvalidValue;  // FP64 companion with 32 amplitudes per thread: not instantiated
   } else {
     if (ta_float) {
       if constexpr (R == 4) {
-        if (phase == 0) return launch_pass_t<float, R, true, 0, float>(d, K, grid, batch, threads, big, st);
-        if (phase == 1) return launch_pass_t<float, R, true, 1, float>(d, K, grid, batch, threads, big, st);
-        return launch_pass_t<float, R, true, 2, float>(d, K, grid, batch, threads, big, st);
+        if (phase == 0) return launch_pass_t<float, R, true, 0, float>(d, K, grid, batch, threads, big, st, occ);
+        if (phase == 1) return launch_pass_t<float, R, true, 1, float>(d, K, grid, batch, threads, big, st, occ);
+        return launch_pass_t<float, R, true, 2, float>(d, K, grid, batch, threads, big, st, occ);
       }
       return hipErrorInvalidValue;
     }
-    if (phase == 0) return launch_pass_t<float, R, true, 0, double>(d, K, grid, batch, threads, big, st);
-    if (phase == 1) return launch_pass_t<float, R, true, 1, double>(d, K, grid, batch, threads, big, st);
-    return launch_pass_t<float, R, true, 2, double>(d, K, grid, batch, threads, big, st);
+    if (phase == 0) return launch_pass_t<float, R, true, 0, double>(d, K, grid, batch, threads, big, st, occ);
+    if (phase == 1) return launch_pass_t<float, R, true, 1, double>(d, K, grid, batch, threads, big, st, occ);
+    return launch_pass_t<float, R, true, 2, double>(d, K, grid, batch, threads, big, st, occ);
   }
 }
 
 hipError_t launch_pass_c64(int R, bool two, int phase, bool ta_float, const PassDesc* d, int K, int grid, int batch,
-                           int threads, bool big, hipStream_t st) {
+                           int threads, bool big, hipStream_t st, int* occ) {
   switch (R) {
-    case 5: return launch_c64_r<5>(two, phase, ta_float, d, K, grid, batch, threads, big, st);
-    case 4: return launch_c64_r<4>(two, phase, ta_float, d, K, grid, batch, threads, big, st);
-    case 3: return launch_c64_r<3>(two, phase, ta_float, d, K, grid, batch, threads, big, st);
-    case 2: return launch_c64_r<2>(two, phase, ta_float, d, K, grid, batch, threads, big, st);
-    case 1: return launch_c64_r<1>(two, phase, ta_float, d, K, grid, batch, threads, big, st);
+    case 5: return launch_c64_r<5>(two, phase, ta_float, d, K, grid, batch, threads, big, st, occ);
+    case 4: return launch_c64_r<4>(two, phase, ta_float, d, K, grid, batch, threads, big, st, occ);
+    case 3: return launch_c64_r<3>(two, phase, ta_float, d, K, grid, batch, threads, big, st, occ);
+    case 2: return launch_c64_r<2>(two, phase, ta_float, d, K, grid, batch, threads, big, st, occ);
+    case 1: return launch_c64_r<1>(two, phase, ta_float, d, K, grid, batch, threads, big, st, occ);
   }
   return hipErrorInvalidValue;
 }

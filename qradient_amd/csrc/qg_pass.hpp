@@ -376,6 +376,96 @@ __device__ __forceinline__ int nnew_of(const int* los, int s, int R) {
   return d < R ? d : R;
 }
 
+// Factored gate on register bit KK (see PassDesc): applies (I - i r P) for
+// form 0 or (r I - i P) for form 1, one FMA per real output; the factor gk is
+// carried by the caller as a running scale.  code = type + 4 * form.
+__device__ __forceinline__ double fmx(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmx(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+template <int R, int KK, class T>
+__device__ __forceinline__ void gate_bit_f(Cx<T> (&x)[1 << R], int code, double rd) {
+  constexpr int A = 1 << R;
+  constexpr int M = 1 << KK;
+  const T r = (T)rd;
+  if (code == G_X) {
+#pragma unroll
+    for (int a = 0; a < A; ++a) {
+      if (a & M) continue;
+      const Cx<T> x0 = x[a], x1 = x[a | M];
+      x[a] = {fmx(r, x1.im, x0.re), fmx(-r, x1.re, x0.im)};
+      x[a | M] = {fmx(r, x0.im, x1.re), fmx(-r, x0.re, x1.im)};
+    }
+  } else if (code == G_X + 4) {
+#pragma unroll
+    for (int a = 0; a < A; ++a) {
+      if (a & M) continue;
+      const Cx<T> x0 = x[a], x1 = x[a | M];
+      x[a] = {fmx(r, x0.re, x1.im), fmx(r, x0.im, -x1.re)};
+      x[a | M] = {fmx(r, x1.re, x0.im), fmx(r, x1.im, -x0.re)};
+    }
+  } else if (code == G_Y) {
+#pragma unroll
+    for (int a = 0; a < A; ++a) {
+      if (a & M) continue;
+      const Cx<T> x0 = x[a], x1 = x[a | M];
+      x[a] = {fmx(-r, x1.re, x0.re), fmx(-r, x1.im, x0.im)};
+      x[a | M] = {fmx(r, x0.re, x1.re), fmx(r, x0.im, x1.im)};
+    }
+  } else if (code == G_Y + 4) {
+#pragma unroll
+    for (int a = 0; a < A; ++a) {
+      if (a & M) continue;
+      const Cx<T> x0 = x[a], x1 = x[a | M];
+      x[a] = {fmx(r, x0.re, -x1.re), fmx(r, x0.im, -x1.im)};
+      x[a | M] = {fmx(r, x1.re, x0.re), fmx(r, x1.im, x0.im)};
+    }
+  } else if (code == G_Z) {
+#pragma unroll
+    for (int a = 0; a < A; ++a) {
+      const Cx<T> v = x[a];
+      // bit 0: (1 - i r) v ; bit 1: (1 + i r) v
+      const T rr = (a & M) ? -r : r;
+      x[a] = {fmx(rr, v.im, v.re), fmx(-rr, v.re, v.im)};
+    }
+  } else if (code == G_Z + 4) {
+#pragma unroll
+    for (int a = 0; a < A; ++a) {
+      const Cx<T> v = x[a];
+      // bit 0: (r - i) v ; bit 1: (r + i) v
+      if (a & M) x[a] = {fmx(r, v.re, -v.im), fmx(r, v.im, v.re)};
+      else x[a] = {fmx(r, v.re, v.im), fmx(r, v.im, -v.re)};
+    }
+  }
+}
+
+// Global addresses (amplitude indices) of a thread's 2^R amplitudes in a tile
+// whose thread-0 register-0 index is g: plain, or through the ladder map.
+template <int R>
+__device__ __forceinline__ void tile_addresses(const PassDesc* __restrict__ d, bool gather, uint32_t g,
+                                               const int (&rpos)[R], const uint32_t (&rbit)[R], int n,
+                                               uint32_t (&adr)[1 << R]) {
+  if (gather) {
+    const uint32_t mbase = apply_map(d->map, g, n);
+#pragma unroll
+    for (int a = 0; a < (1 << R); ++a) {
+      uint32_t v = mbase;
+#pragma unroll
+      for (int k = 0; k < R; ++k)
+        if (a & (1 << k)) v ^= d->map[rpos[k]];
+      adr[a] = v;
+    }
+  } else {
+#pragma unroll
+    for (int a = 0; a < (1 << R); ++a) {
+      uint32_t v = g;
+#pragma unroll
+      for (int k = 0; k < R; ++k)
+        if (a & (1 << k)) v |= rbit[k];
+      adr[a] = v;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // One pass over the tiles assigned to this workgroup.
 //   S  : storage and LDS element type (float: complex64, double: complex128)
@@ -383,11 +473,21 @@ __device__ __forceinline__ int nnew_of(const int* los, int s, int R) {
 //   TWO: an adjoint pass with the companion state A and the propagated B
 //   PH : diagonal phase method (0 none, 1 table, 2 sincos)
 //   TA : register precision of A in a TWO pass (the propagated state is FP64)
+//
+// The next tile's loads are issued before the current tile is processed (the
+// raw storage-type values wait in registers), except in complex128 adjoint
+// passes where that would double the register footprint.  In complex64
+// adjoint passes the reductions are accumulated per thread across all tiles in
+// LDS slots [local bit][thread] and reduced across the workgroup once, at the
+// end; complex128 adjoint passes reduce per tile into per-wave slots.  Both
+// use a fixed order, so results are bitwise reproducible.
 // ---------------------------------------------------------------------------
 template <class S, int R, bool TWO, int PH, class TA, bool BIG>
 __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* lds, uint32_t tid_in, uint32_t blk,
                                          uint32_t nblk) {
   constexpr int A = 1 << R;
+  constexpr bool PF = !(TWO && sizeof(S) == 8);
+  constexpr bool ACCL = TWO && sizeof(S) == 4;
   typedef typename std::conditional<TWO, TA, double>::type TAr;
   const int n = d->n, K = d->K, t = d->t, c = d->c, p = d->p;
   const uint32_t flags = d->flags;
@@ -417,77 +517,85 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
     rbit[k] = 1u << rpos[k];
   }
 
-  // Per-wave FP64 accumulators in LDS after the tile buffer: [wave][K + 1];
-  // each (wave, slot) has one writer (lane 0), so the summation order is fixed.
+  // LDS after the tile buffer: per-wave FP64 slots [wave][K + 1] (lane 0 of
+  // each wave is the only writer of its row), then for ACCL the per-thread
+  // accumulators [local bit][thread].
   double* wacc = reinterpret_cast<double*>(reinterpret_cast<char*>(lds) + (sizeof(S) * 2u << K));
+  double* tacc = wacc + nwaves * (kMaxK + 1);
   const bool reduce = TWO && (flags & (F_REDUCE | F_DRED_POST | F_OBS_DIAG)) && d->row_base >= 0;
+  const bool reduce_bits = reduce && (flags & F_REDUCE);
+  const uint32_t wave = tid_in >> 6;
+  const bool lane0 = (tid_in & 63u) == 0u;
   if (reduce) {
     for (uint32_t i = tid_in; i < (uint32_t)(nwaves * (K + 1)); i += nthr) wacc[i] = 0.0;
+    if constexpr (ACCL) {
+      for (int lb = 0; lb < K; ++lb) tacc[lb * nthr + tid_in] = 0.0;
+    }
     __syncthreads();
   }
+  double dacc = 0.0;
+  const bool gather = (flags & F_GATHER) != 0;
+  const bool load_a = !(flags & F_INIT_TAB);
+  const bool load_b = TWO && (flags & F_LOAD_B);
+  // A's gates only matter for later reductions or for A's own use after the gates
+  const bool a_used_after = !TWO || (flags & (F_STORE_A | F_DRED_POST | F_OBS_DIAG));
 
   // XCD-aware tile order: workgroup blk runs on XCD blk % 8; each XCD gets a
   // contiguous range of tiles so tiles that share cache lines share an L2.
   const bool xcd = (nblk >= 8) && (nblk % 8 == 0) && (ntiles % nblk == 0);
   const uint32_t per_x = ntiles / 8, slots = nblk / 8;
-  const uint32_t rounds = (ntiles + nblk - 1) / nblk;
+  const uint32_t rounds = xcd ? ntiles / nblk : (blk < ntiles ? (ntiles - 1 - blk) / nblk + 1 : 0u);
+  auto tile_of = [&](uint32_t r) -> uint32_t {
+    return xcd ? (blk % 8) * per_x + r * slots + blk / 8 : blk + r * nblk;
+  };
 
-  for (uint32_t r = 0; r < rounds; ++r) {
-    // Recompute thread-index-derived addresses inside the loop rather than
-    // hoisting loop-invariant offsets into registers.
+  Cx<S> ra[A];
+  Cx<S> rb[TWO ? A : 1];
+  // Issues the loads of tile tt into ra / rb (addresses derived from a
+  // laundered thread index so they are not kept live across the tile).
+  auto fetch = [&](uint32_t tt) __attribute__((always_inline)) {
     uint32_t tid = tid_in;
     asm volatile("" : "+v"(tid));
-    uint32_t tt;
-    if (xcd) {
-      tt = (blk % 8) * per_x + r * slots + blk / 8;
-    } else {
-      tt = blk + r * nblk;
-      if (tt >= ntiles) break;
-    }
-    const uint32_t gthr = gl_tile(tt, c, p, K) | gl_local(tid, c, p);
-
-    // ---- addresses and loads ------------------------------------------------
     uint32_t adr[A];
-    if (flags & F_GATHER) {
-      const uint32_t mbase = apply_map(d->map, gthr, n);
-#pragma unroll
-      for (int a = 0; a < A; ++a) {
-        uint32_t v = mbase;
-#pragma unroll
-        for (int k = 0; k < R; ++k)
-          if (a & (1 << k)) v ^= d->map[rpos[k]];
-        adr[a] = v;
-      }
-    } else {
-#pragma unroll
-      for (int a = 0; a < A; ++a) {
-        uint32_t v = gthr;
-#pragma unroll
-        for (int k = 0; k < R; ++k)
-          if (a & (1 << k)) v |= rbit[k];
-        adr[a] = v;
-      }
-    }
-    Cx<TAr> xa[A];
-    Cd xb[TWO ? A : 1];
-    if (flags & F_INIT_TAB) {
-#pragma unroll
-      for (int a = 0; a < A; ++a) xa[a] = {(TAr)d->init_tab[__popc(adr[a])], (TAr)0};
-    } else {
+    tile_addresses<R>(d, gather, gl_tile(tt, c, p, K) | gl_local(tid, c, p), rpos, rbit, n, adr);
+    if (load_a) {
 #pragma unroll
       for (int a = 0; a < A; ++a)
-        xa[a] = BIG ? Stor<S>::template ld_big<TAr>(d->a_in, adr[a]) : Stor<S>::template ld<TAr>(d->a_in, adr[a]);
+        ra[a] = BIG ? Stor<S>::template ld_big<S>(d->a_in, adr[a]) : Stor<S>::template ld<S>(d->a_in, adr[a]);
     }
     if constexpr (TWO) {
-      if (flags & F_LOAD_B) {
+      if (load_b) {
 #pragma unroll
         for (int a = 0; a < A; ++a)
-          xb[a] = BIG ? Stor<S>::template ld_big<double>(d->b_in, adr[a]) : Stor<S>::template ld<double>(d->b_in, adr[a]);
-      } else {
-#pragma unroll
-        for (int a = 0; a < A; ++a) xb[a] = {0.0, 0.0};
+          rb[a] = BIG ? Stor<S>::template ld_big<S>(d->b_in, adr[a]) : Stor<S>::template ld<S>(d->b_in, adr[a]);
       }
     }
+  };
+
+  if (PF && rounds > 0) fetch(tile_of(0));
+  for (uint32_t r = 0; r < rounds; ++r) {
+    const uint32_t tt = tile_of(r);
+    if (!PF) fetch(tt);
+    uint32_t tid = tid_in;
+    asm volatile("" : "+v"(tid));
+    const uint32_t gthr = gl_tile(tt, c, p, K) | gl_local(tid, c, p);
+
+    Cx<TAr> xa[A];
+    Cd xb[TWO ? A : 1];
+    if (load_a) {
+#pragma unroll
+      for (int a = 0; a < A; ++a) xa[a] = {(TAr)ra[a].re, (TAr)ra[a].im};
+    } else {
+      uint32_t adr[A];
+      tile_addresses<R>(d, gather, gthr, rpos, rbit, n, adr);
+#pragma unroll
+      for (int a = 0; a < A; ++a) xa[a] = {(TAr)d->init_tab[__popc(adr[a])], (TAr)0};
+    }
+    if constexpr (TWO) {
+#pragma unroll
+      for (int a = 0; a < A; ++a) xb[a] = load_b ? Cd{(double)rb[a].re, (double)rb[a].im} : Cd{0.0, 0.0};
+    }
+    if (PF && r + 1 < rounds) fetch(tile_of(r + 1));
 
     // ---- diagonal ops in the natural layout, before the gates --------------
     if (flags & F_CZ_PRE) {
@@ -511,43 +619,68 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
     }
 
     // ---- stages: reductions then gates on each newly visible register bit --
+    // kA / kB: running products of the gate factors applied to A / B
+    double kA = 1.0, kB = 1.0;
 #pragma unroll 1
-    for (int s = 0; s < ns; ++s) {
-      if (s > 0) {
-        lds_transpose<S, R>(xa, lds, tid, los[s - 1], los[s]);
-        if constexpr (TWO) lds_transpose<S, R>(xb, lds, tid, los[s - 1], los[s]);
-      }
-      const int lo = los[s];
-      const int nnew = nnew_of(los, s, R);
-      const bool last_stage = (s == ns - 1);
-      static_for<R>([&](auto kc) __attribute__((always_inline)) {
-        constexpr int kk = decltype(kc)::value;
-        if (kk < nnew) {
-          // opaque to the optimiser: keeps the uniform gate loads next to
-          // their use instead of hoisting them out of the loops
-          int lb = __builtin_amdgcn_readfirstlane(lo + kk);
-          asm volatile("" : "+s"(lb));
-          const int ty = d->gtype[lb];
-          if constexpr (TWO) {
-            if ((flags & F_REDUCE) && d->own[lb] && d->row_base >= 0) {
-              const double v = wave_sum(red_bit<R, kk>(xb, xa, ty), width);
-              if ((tid & 63) == 0) wacc[(tid >> 6) * (K + 1) + lb] += v;
+    for (int s = 0; s < kMaxStages; ++s) {
+      if (s < ns) {
+        if (s > 0) {
+          lds_transpose<S, R>(xa, lds, tid, los[s - 1], los[s]);
+          if constexpr (TWO) lds_transpose<S, R>(xb, lds, tid, los[s - 1], los[s]);
+        }
+        const int lo = los[s];
+        const int nnew = nnew_of(los, s, R);
+        const bool last_stage = (s == ns - 1);
+        static_for<R>([&](auto kc) __attribute__((always_inline)) {
+          constexpr int kk = decltype(kc)::value;
+          if (kk < nnew) {
+            // opaque to the optimiser: keeps the uniform gate loads next to
+            // their use instead of hoisting them out of the loops
+            int lb = __builtin_amdgcn_readfirstlane(lo + kk);
+            asm volatile("" : "+s"(lb));
+            const int code = d->gtype[lb];
+            if constexpr (TWO) {
+              if (reduce_bits && d->own[lb]) {
+                const double v = red_bit<R, kk>(xb, xa, code & 3) * (kA * kB);
+                if constexpr (ACCL) {
+                  tacc[lb * nthr + tid] += v;
+                } else {
+                  const double w = wave_sum(v, width);
+                  if ((tid & 63) == 0) wacc[(tid >> 6) * (K + 1) + lb] += w;
+                }
+              }
+            }
+            const double gr = d->gr[lb], gk = d->gk[lb];
+            const bool need_a = a_used_after || !last_stage || kk < nnew - 1;
+            if ((flags & F_ROT_A) && need_a) {
+              gate_bit_f<R, kk>(xa, code, gr);
+              kA *= gk;
+            }
+            if constexpr (TWO) {
+              if (flags & F_ROT_B) {
+                gate_bit_f<R, kk>(xb, code, gr);
+                kB *= gk;
+              }
             }
           }
-          // A's gates only matter for later reductions (the remaining bits of
-          // this stage and all later stages) or its store
-          const bool need_a =
-              !TWO || (flags & F_STORE_A) || (flags & F_DRED_POST) || !last_stage || kk < nnew - 1;
-          if ((flags & F_ROT_A) && need_a) gate_bit<R, kk>(xa, ty, d->gc[lb], d->gs[lb]);
-          if constexpr (TWO) {
-            if (flags & F_ROT_B) gate_bit<R, kk>(xb, ty, d->gc[lb], d->gs[lb]);
-          }
-        }
-      });
+        });
+      }
     }
     if (ns > 1) {
       lds_transpose<S, R>(xa, lds, tid, los[ns - 1], t);
       if constexpr (TWO) lds_transpose<S, R>(xb, lds, tid, los[ns - 1], t);
+    }
+    // restore the gate factors
+    if (a_used_after && kA != 1.0) {
+      const TAr ka = (TAr)kA;
+#pragma unroll
+      for (int a = 0; a < A; ++a) xa[a] = {ka * xa[a].re, ka * xa[a].im};
+    }
+    if constexpr (TWO) {
+      if (kB != 1.0) {
+#pragma unroll
+        for (int a = 0; a < A; ++a) xb[a] = {kB * xb[a].re, kB * xb[a].im};
+      }
     }
 
     // ---- diagonal ops in the natural layout, after the gates --------------
@@ -576,10 +709,7 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
           }
         }
       }
-      if (reduce && (flags & (F_DRED_POST | F_OBS_DIAG))) {
-        const double v = wave_sum(xacc, width);
-        if ((tid & 63) == 0) wacc[(tid >> 6) * (K + 1) + K] += v;
-      }
+      dacc += xacc;
     }
     if (flags & F_CZ_POST) {
 #pragma unroll
@@ -597,26 +727,8 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
     // ---- store: addresses recomputed from a laundered base, not kept live ----
     uint32_t gst = gthr;
     asm volatile("" : "+v"(gst));
-    if (flags & F_SCATTER) {
-      const uint32_t mbase = apply_map(d->map, gst, n);
-#pragma unroll
-      for (int a = 0; a < A; ++a) {
-        uint32_t v = mbase;
-#pragma unroll
-        for (int k = 0; k < R; ++k)
-          if (a & (1 << k)) v ^= d->map[rpos[k]];
-        adr[a] = v;
-      }
-    } else {
-#pragma unroll
-      for (int a = 0; a < A; ++a) {
-        uint32_t v = gst;
-#pragma unroll
-        for (int k = 0; k < R; ++k)
-          if (a & (1 << k)) v |= rbit[k];
-        adr[a] = v;
-      }
-    }
+    uint32_t adr[A];
+    tile_addresses<R>(d, (flags & F_SCATTER) != 0, gst, rpos, rbit, n, adr);
     if (flags & F_STORE_A) {
 #pragma unroll
       for (int a = 0; a < A; ++a) {
@@ -635,11 +747,22 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
     }
   }
 
-  // ---- fixed-order workgroup reduction: sum the per-wave slots in order -------
+  // ---- fixed-order reduction: per wave (shuffle tree), then the waves in order
   if (!reduce) return;
+  if constexpr (ACCL) {
+    for (int lb = 0; lb < K; ++lb) {
+      if (reduce_bits && d->own[lb]) {
+        const double v = wave_sum(tacc[lb * nthr + tid_in], width);
+        if (lane0) wacc[wave * (K + 1) + lb] = v;
+      }
+    }
+  }
+  if (flags & (F_DRED_POST | F_OBS_DIAG)) {
+    const double v = wave_sum(dacc, width);
+    if (lane0) wacc[wave * (K + 1) + K] = v;
+  }
   __syncthreads();
-  const uint32_t tid = tid_in;
-  for (uint32_t i = tid; i < (uint32_t)(K + 1); i += nthr) {
+  for (uint32_t i = tid_in; i < (uint32_t)(K + 1); i += nthr) {
     int row = -1;
     if (i == (uint32_t)K) {
       if (flags & (F_DRED_POST | F_OBS_DIAG)) row = d->row_base + K;
@@ -662,15 +785,25 @@ __global__ __launch_bounds__(512) void k_pass(const PassDesc* __restrict__ d) {
   run_pass<S, R, TWO, PH, TA, BIG>(d + blockIdx.y, lds_dyn, threadIdx.x, blockIdx.x, gridDim.x);
 }
 
-// LDS bytes of a pass: the tile in the storage type plus the per-wave slots.
-__host__ __device__ inline size_t pass_lds_bytes(int dtype, int K, int threads) {
-  return (size_t)(1u << K) * (dtype == 1 ? 8 : 16) + (size_t)((threads + 63) / 64) * (kMaxK + 1) * sizeof(double);
+// LDS bytes of a pass: the tile in the storage type, the per-wave slots and,
+// for complex64 adjoint passes, the per-thread accumulators.
+__host__ __device__ inline size_t pass_lds_bytes(int dtype, int K, int threads, bool two) {
+  return (size_t)(1u << K) * (dtype == 1 ? 8 : 16) + (size_t)((threads + 63) / 64) * (kMaxK + 1) * sizeof(double) +
+         (two && dtype == 1 ? (size_t)K * threads * sizeof(double) : 0);
 }
 
 // BIG: the state is 4 GiB or larger, so byte offsets need 64-bit addresses.
+// With occ != NULL nothing is launched: *occ receives the number of
+// workgroups of this instantiation that fit one CU.
 template <class S, int R, bool TWO, int PH, class TA>
-hipError_t launch_pass_t(const PassDesc* d, int K, int grid, int batch, int threads, bool big, hipStream_t st) {
-  const size_t lds = pass_lds_bytes(sizeof(S) == 4 ? 1 : 0, K, threads);
+hipError_t launch_pass_t(const PassDesc* d, int K, int grid, int batch, int threads, bool big, hipStream_t st,
+                         int* occ) {
+  const size_t lds = pass_lds_bytes(sizeof(S) == 4 ? 1 : 0, K, threads, TWO);
+  if (occ) {
+    if (big)
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_pass<S, R, TWO, PH, TA, true>, threads, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_pass<S, R, TWO, PH, TA, false>, threads, lds);
+  }
   if (big)
     hipLaunchKernelGGL((k_pass<S, R, TWO, PH, TA, true>), dim3(grid, batch), dim3(threads), lds, st, d);
   else
@@ -680,7 +813,7 @@ hipError_t launch_pass_t(const PassDesc* d, int K, int grid, int batch, int thre
 
 template <class S, int R, bool TWO, int PH, class TA>
 hipError_t set_attr_t() {
-  const int maxlds = (int)pass_lds_bytes(sizeof(S) == 4 ? 1 : 0, kMaxK, 512);
+  const int maxlds = (int)pass_lds_bytes(sizeof(S) == 4 ? 1 : 0, kMaxK, 512, TWO);
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pass<S, R, TWO, PH, TA, false>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, maxlds);
   if (e == hipSuccess)

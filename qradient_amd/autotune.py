@@ -57,7 +57,7 @@ def candidates(dtype):
     out = []
     for kmax in (13, 12, 11):
         for cmin in (2, 3, 4):
-            rs = (4, 5) if dtype == 'complex64' else (4,)
+            rs = (4,)
             for r_fwd in rs:
                 for r_rev in rs:
                     out.append({'kmax': kmax, 'cmin': cmin, 'r_fwd': r_fwd, 'r_rev': r_rev})

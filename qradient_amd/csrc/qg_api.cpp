@@ -15,8 +15,8 @@
 
 namespace qg {
 hipError_t set_pass_attributes();
-hipError_t launch_pass(int dtype, int R, bool two, int phase, bool ta_float, const PassDesc* d, int K, int grid,
-                       int batch, int threads, int n, hipStream_t st, int* occ);
+hipError_t launch_pass(int dtype, int R, bool two, int phase, bool post, const PassDesc* d, int K, int grid,
+                       int batch, int threads, int n, hipStream_t st, int* occ, int tab);
 hipError_t launch_finalize(int nout, const int* ptr, const int* rows, const int* cnt, const double* scale,
                            const double* part, int stride, double* out, hipStream_t st);
 hipError_t simple_rot1(int dtype, void* v, int n, int b, int type, double c, double s, hipStream_t st);
@@ -226,7 +226,7 @@ static std::vector<Geo> layer_geometry(int n, int kmax, int cmin) {
 // FP32 registers); passes that propagate two FP64 states hold 16.
 enum Role { ROLE_FWD = 0, ROLE_REV_HIST = 1, ROLE_TWO_FP64 = 2 };
 
-static Geo with_role_r(const Geo& g0, int dtype, int role, int r_fwd, int r_rev, bool* ta_float) {
+static Geo with_role_r(const Geo& g0, int dtype, int role, int r_fwd, int r_rev, bool* post) {
   Geo g = g0;
   int R = 4;
   if (dtype == QG_C64 && role == ROLE_FWD) R = r_fwd;
@@ -234,8 +234,8 @@ static Geo with_role_r(const Geo& g0, int dtype, int role, int r_fwd, int r_rev,
   g.R = std::min(R, g.K);
   g.t = g.K - g.R;
   g.threads = 1 << g.t;
-  // a complex64 history-adjoint pass holds its read-only companion in FP32
-  *ta_float = (dtype == QG_C64 && role == ROLE_REV_HIST && g.R >= 4);
+  // the kernel variant with post-gate diagonal ops is chosen per launch
+  *post = false;
   return g;
 }
 #define with_role(g0, dtype, role, tf) with_role_r(g0, dtype, role, p->r_fwd, p->r_rev, tf)
@@ -270,7 +270,8 @@ struct Launch {
   int desc;  // index into the descriptor table (first of `batch` consecutive ones)
   int batch = 1;
   int K = 0;
-  bool ta_float = false;  // FP32 companion registers (complex64 history adjoint pass)
+  bool post = false;  // the pass has diagonal ops after its gates (set at launch from the flags)
+  int tab = 0;        // bytes of the table the pass stages in LDS (set at launch)
   int R, threads, grid, phase;
   bool two;
   int kind;  // 0 pass kernel, 1 generic observable, 2 convert-in (ini state)
@@ -329,6 +330,7 @@ struct qg_plan {
   std::vector<float> pass_ms;  // durations of the last profiled run
   std::vector<int> pass_kind;  // 0 forward, 1 reverse (two-state), 2 forward+observable
   int last_state = -1;  // 0: final state in hist, used by get_state
+  bool state_sign = false;  // the pass kernels' final state is -1 times the circuit's
   void* last_state_ptr = nullptr;
 };
 
@@ -355,9 +357,9 @@ static size_t lds_bytes_of(int dtype, int K, int threads) {  // the larger (adjo
 // Workgroups per pass launch: as many as are resident at once on the device
 // (occupancy of the kernel variant, by its registers and LDS), rounded down to
 // a power of two and at most one per tile.
-static int grid_for_pass(const qg_plan* p, const Geo& g, bool two, bool ta_float, int phase) {
+static int grid_for_pass(const qg_plan* p, const Geo& g, bool two, bool post, int phase) {
   int occ = 0;
-  if (launch_pass(p->dtype, g.R, two, phase, ta_float, nullptr, g.K, 0, 1, g.threads, p->n, nullptr, &occ) !=
+  if (launch_pass(p->dtype, g.R, two, phase, post, nullptr, g.K, 0, 1, g.threads, p->n, nullptr, &occ, 0) !=
           hipSuccess ||
       occ < 1) {
     const size_t lds = lds_bytes_of(p->dtype, g.K, g.threads);
@@ -380,8 +382,8 @@ static int grid_max(const qg_plan* p) {
       for (int phase = 0; phase < 3; ++phase) {
         bool tf;
         const Geo G = with_role(g, p->dtype, role, &tf);
-        gm = std::max(gm, grid_for_pass(p, G, role != ROLE_FWD, tf, phase));
-        if (role == ROLE_FWD) gm = std::max(gm, grid_for_pass(p, G, true, false, phase));
+        gm = std::max(gm, grid_for_pass(p, G, role != ROLE_FWD, false, phase));
+        gm = std::max(gm, grid_for_pass(p, G, true, true, phase));
       }
   return gm;
 }
@@ -456,6 +458,8 @@ extern "C" int qg_plan_get_state(qg_plan* p, double* out) {
   if (e == hipSuccess) e = hipStreamSynchronize(p->ctx->stream);
   hipFree(tmp);
   if (e != hipSuccess) return fail(QG_ERR_HIP, std::string("qg_plan_get_state: ") + hipGetErrorString(e));
+  if (p->state_sign)
+    for (size_t k = 0; k < 2 * N; ++k) out[k] = -out[k];
   return QG_OK;
 }
 
@@ -472,30 +476,90 @@ static void desc_geo(PassDesc& d, int n, const Geo& g) {
   d.row_base = -1;
   for (int k = 0; k < kMaxK; ++k) {
     d.gtype[k] = G_NONE;
-    d.gr[k] = 0.0;
-    d.gk[k] = 1.0;
+    d.ga[k] = 0.0;
+    d.gb[k] = 0.0;
   }
 }
 
-// U = c I - i s P with c = cos(angle/2), s = sign * sin(angle/2), stored in the
-// factored form the pass kernels apply (one FMA per real output): the larger
-// of |c|, |s| is the factor, so the ratio stays in [-1, 1].
-static void set_gate(PassDesc& d, int lb, int axis, double angle, double sign) {
-  const int type = axis == 0 ? G_X : axis == 1 ? G_Y : G_Z;
-  const double c = std::cos(0.5 * angle), s = sign * std::sin(0.5 * angle);
-  if (std::fabs(c) >= std::fabs(s)) {
-    d.gtype[lb] = type;
-    d.gk[lb] = c;
-    d.gr[lb] = s / c;
-  } else {
-    d.gtype[lb] = type + 4;
-    d.gk[lb] = s;
-    d.gr[lb] = c / s;
+// U = cos(phi) I - i sin(phi) P with phi = sign * angle / 2.  phi is reduced
+// modulo pi to [-pi/2, pi/2]; each reduction multiplies U by -1, a global sign
+// the kernels drop (returns true when it did, so the caller can track the
+// parity for the returned state vector).
+static bool gate_flips(double angle, double sign) {
+  const double phi0 = sign * 0.5 * angle;
+  const double phi = std::remainder(phi0, M_PI);
+  return std::fabs(std::remainder((phi0 - phi) / M_PI, 2.0)) > 0.5;
+}
+
+static bool set_gate(PassDesc& d, int lb, int axis, double angle, double sign) {
+  const double phi0 = sign * 0.5 * angle;
+  const double phi = std::remainder(phi0, M_PI);
+  const bool flipped = gate_flips(angle, sign);
+  const double c = std::cos(phi), s = std::sin(phi);
+  d.gtype[lb] = axis == 0 ? G_X : axis == 1 ? G_Y : G_Z;
+  d.ga[lb] = s / (1.0 + c);
+  d.gb[lb] = s;
+  return flipped;
+}
+
+// Fills the per-stage-slot gate tables of a descriptor from its per-bit
+// arrays, for a launch holding 2^R amplitudes per thread (the kernel's stage
+// layout: stage 0 holds local bits [t, t + R), stage s >= 1 holds
+// [max(0, t - R s), +R); a bit is gated in the first stage that holds it).
+static void desc_slots(PassDesc& d, int R, bool two) {
+  int los[kMaxStages];
+  int ns = 1;
+  los[0] = d.t;
+  for (int s = 1; s < kMaxStages; ++s) {
+    los[s] = 0;
+    if (ns == s && los[s - 1] > 0) {
+      los[s] = std::max(0, d.t - R * s);
+      ns = s + 1;
+    }
   }
+  for (int k = 0; k < kMaxSlots; ++k) {
+    d.stype[k] = G_NONE;
+    d.sa[k] = d.sb[k] = 0.0;
+    d.sflag[k] = 0;
+    d.slb[k] = 0;
+  }
+  const bool a_used_after = !two || (d.flags & (F_STORE_A | F_DRED_POST | F_OBS_DIAG));
+  int last_a = -1;
+  for (int s = 0; s < ns; ++s) {
+    const int nnew = s == 0 ? R : std::min(los[s - 1] - los[s], R);
+    for (int kk = 0; kk < nnew; ++kk) {
+      const int slot = s * R + kk, lb = los[s] + kk;
+      d.slb[slot] = lb;
+      d.stype[slot] = (two || (d.flags & F_ROT_A)) ? d.gtype[lb] : G_NONE;
+      d.sa[slot] = d.ga[lb];
+      d.sb[slot] = d.gb[lb];
+      int f = 0;
+      if (d.gtype[lb] != G_NONE) {
+        if (d.flags & F_ROT_A) {
+          f |= SL_ROT_A;
+          last_a = slot;
+        }
+        if (two && (d.flags & F_ROT_B)) f |= SL_ROT_B;
+      }
+      if (two && (d.flags & F_REDUCE) && d.own[lb] && d.row_base >= 0) f |= SL_RED;
+      d.sflag[slot] = f;
+    }
+  }
+  // A's last gate matters only if A is used after the gates
+  if (!a_used_after && last_a >= 0) d.sflag[last_a] &= ~SL_ROT_A;
 }
 
 static int run_launches(qg_plan* p, int ndesc) {
   qg_ctx* c = p->ctx;
+  for (Launch& l : p->launches)
+    if (l.kind == 0) {
+      for (int b = 0; b < l.batch; ++b) desc_slots(p->h_desc[l.desc + b], l.R, l.two);
+      const PassDesc& d = p->h_desc[l.desc];
+      l.post = l.two && (d.flags & (F_DRED_POST | F_PHASE_POST | F_OBS_DIAG)) != 0;
+      l.tab = ((d.flags & F_INIT_TAB) ? (int)sizeof(double) * (p->n + 1) : 0) +
+              ((l.phase == 1 && (d.flags & (F_PHASE_PRE | F_PHASE_POST))) ? (int)sizeof(double) * 2 * (2 * d.lut_off + 1)
+                                                                           : 0);
+    }
   HIPCHK(hipMemcpyAsync(p->d_desc, p->h_desc, sizeof(PassDesc) * ndesc, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   const size_t nl = p->launches.size();
@@ -510,8 +574,8 @@ static int run_launches(qg_plan* p, int ndesc) {
     const Launch& l = p->launches[k];
     if (p->profile) HIPCHK(hipEventRecord(p->pev[k], c->stream));
     if (l.kind == 0) {
-      HIPCHK(launch_pass(p->dtype, l.R, l.two, l.phase, l.ta_float, p->d_desc + l.desc, l.K, l.grid, l.batch,
-                         l.threads, p->n, c->stream, nullptr));
+      HIPCHK(launch_pass(p->dtype, l.R, l.two, l.phase, l.post, p->d_desc + l.desc, l.K, l.grid, l.batch,
+                         l.threads, p->n, c->stream, nullptr, l.tab));
     }
   }
   if (p->profile) HIPCHK(hipEventRecord(p->pev[nl], c->stream));
@@ -698,7 +762,7 @@ static int mcclean_schedule(qg_plan* p, const int32_t* axes, const double* angle
     l.desc = nd;
     l.R = g.R;
     l.K = g.K;
-    l.ta_float = tf;
+    l.post = tf;
     l.threads = g.threads;
     l.grid = grid_for_pass(p, g, two, tf, 0);
     l.phase = 0;
@@ -777,6 +841,8 @@ static int mcclean_schedule(qg_plan* p, const int32_t* axes, const double* angle
     }
   }
   p->last_state_ptr = (void*)cur;
+  p->state_sign = false;
+  for (int k = 0; k < L * n; ++k) p->state_sign ^= gate_flips(angles[k], 1.0);
   if (!grad) {
     ndesc_out = nd;
     return QG_OK;
@@ -973,6 +1039,7 @@ static int mcclean_run(qg_plan* p, const int32_t* axes, const double* angles, co
     *E_out = res[0];
     if (grad) std::memcpy(grad_out, res.data() + 1, sizeof(double) * L * n);
     p->last_state_ptr = p->d_fstate;
+    p->state_sign = false;
     return QG_OK;
   }
   if (!obs.xy.empty())
@@ -1127,7 +1194,8 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
     }
     W = std::max(W, wsum);
   }
-  const int phase = (integral && W <= 4096.0) ? 1 : 2;
+  // integer weights: exp(-i gamma C) from a table of 2W + 1 entries staged in LDS
+  const int phase = (integral && 2.0 * W + 1.0 <= (double)kLutMax) ? 1 : 2;
   const int Wi = (int)W;
   qg_ctx* c = p->ctx;
   HIPCHK(hipSetDevice(c->device));
@@ -1220,7 +1288,7 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
       l.batch = nb;
       l.R = G.R;
       l.K = G.K;
-      l.ta_float = tf;
+      l.post = tf;
       l.threads = G.threads;
       l.grid = grid_for_pass(p, G, two, tf, phase);
       l.phase = phase;
@@ -1324,6 +1392,9 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
         for (int k = 0; k < 2 * L; ++k) grad_out[(size_t)(c0 + j) * 2 * L + k] = o[1 + k];
     }
     p->last_state_ptr = hist(0, L - 1, P - 1);
+    p->state_sign = false;
+    for (int i = 0; i < L; ++i)
+      if (n % 2 == 1) p->state_sign ^= gate_flips(betas[(size_t)c0 * L + i], 1.0);
   }
   p->last_ms = total_ms;
   return QG_OK;
@@ -1584,7 +1655,7 @@ extern "C" int qg_plan_tune(qg_plan* p, int kmax, int cmin, int r_fwd, int r_rev
   if (!p) return fail(QG_ERR_INVALID, "NULL plan");
   if (kmax < 4 || kmax > kMaxK) return fail(QG_ERR_INVALID, "kmax must be in [4, 13]");
   if (cmin < 0 || cmin >= kmax) return fail(QG_ERR_INVALID, "cmin must be in [0, kmax)");
-  if (r_fwd < 4 || r_fwd > 5 || r_rev < 4 || r_rev > 5) return fail(QG_ERR_INVALID, "r_fwd, r_rev must be 4 or 5");
+  if (r_fwd != 4 || r_rev != 4) return fail(QG_ERR_INVALID, "r_fwd, r_rev must be 4 (16 amplitudes per thread)");
   if (p->kind != PK_MCCLEAN) return fail(QG_ERR_UNSUPPORTED, "tuning applies to McClean plans");
   HIPCHK(hipSetDevice(p->ctx->device));
   HIPCHK(hipStreamSynchronize(p->ctx->stream));

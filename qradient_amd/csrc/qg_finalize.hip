@@ -6,10 +6,10 @@
 
 namespace qg {
 
-hipError_t launch_pass_c64(int R, bool two, int phase, bool ta_float, const PassDesc* d, int K, int grid, int batch,
-                           int threads, bool big, hipStream_t st, int* occ);
-hipError_t launch_pass_c128(int R, bool two, int phase, const PassDesc* d, int K, int grid, int batch, int threads,
-                            bool big, hipStream_t st, int* occ);
+hipError_t launch_pass_c64(int R, bool two, int phase, bool post, const PassDesc* d, int K, int grid, int batch,
+                           int threads, bool big, hipStream_t st, int* occ, int tab);
+hipError_t launch_pass_c128(int R, bool two, int phase, bool post, const PassDesc* d, int K, int grid, int batch,
+                            int threads, bool big, hipStream_t st, int* occ, int tab);
 hipError_t set_attributes_c64();
 hipError_t set_attributes_c128();
 
@@ -49,15 +49,16 @@ hipError_t set_pass_attributes() {
   return e;
 }
 
-// dtype 1: complex64 storage, 0: complex128.  ta_float selects the FP32
-// companion registers of a complex64 adjoint pass.  With occ != NULL the
-// pass is not launched: *occ receives the workgroups per CU.
-hipError_t launch_pass(int dtype, int R, bool two, int phase, bool ta_float, const PassDesc* d, int K, int grid,
-                       int batch, int threads, int n, hipStream_t st, int* occ) {
+// dtype 1: complex64 storage, 0: complex128.  post: the pass has diagonal
+// ops after its gates.  With occ != NULL the pass is not launched: *occ
+// receives the workgroups per CU.  tab: bytes of the table the pass stages in
+// LDS.
+hipError_t launch_pass(int dtype, int R, bool two, int phase, bool post, const PassDesc* d, int K, int grid,
+                       int batch, int threads, int n, hipStream_t st, int* occ, int tab) {
   // byte offsets of a 2^n state: 32-bit addressing below 4 GiB
   const bool big = (n + (dtype == 1 ? 3 : 4)) > 32;
-  if (dtype == 1) return launch_pass_c64(R, two, phase, ta_float, d, K, grid, batch, threads, big, st, occ);
-  return launch_pass_c128(R, two, phase, d, K, grid, batch, threads, big, st, occ);
+  if (dtype == 1) return launch_pass_c64(R, two, phase, post, d, K, grid, batch, threads, big, st, occ, tab);
+  return launch_pass_c128(R, two, phase, post, d, K, grid, batch, threads, big, st, occ, tab);
 }
 
 }  // namespace qg

@@ -12,6 +12,10 @@ constexpr int kMaxK = 13;        // max local (tile) bits of a pass: 2^13 FP64 c
 constexpr int kRegBits = 4;      // amplitudes held per thread = 2^kRegBits
 constexpr int kMaxStages = 4;    // natural stage + ceil((kMaxK - kRegBits) / kRegBits)
 constexpr int kMaxQubits = 32;     // 32-bit amplitude indices
+constexpr int kMaxR = 5;           // log2 amplitudes per thread
+constexpr int kMaxSlots = kMaxStages * kMaxR;
+enum : int32_t { SL_ROT_A = 1, SL_ROT_B = 2, SL_RED = 4 };
+constexpr int kLutMax = 1024;      // phase-table entries staged in LDS (2W + 1 <= kLutMax)
 
 enum GateType : int32_t { G_NONE = 0, G_X = 1, G_Y = 2, G_Z = 3 };
 
@@ -53,14 +57,24 @@ struct PassDesc {
   void* a_out;
   const void* b_in;
   void* b_out;
-  // gates, indexed by local bit, in factored form (inverses have s negated):
-  //   U = c I - i s P = gk * (I - i gr P)   (form 0, |c| >= |s|: gk = c, gr = s / c)
-  //                   = gk * (gr I - i P)   (form 1, |s| >  |c|: gk = s, gr = c / s)
-  // gtype = G_X / G_Y / G_Z plus 4 * form; G_NONE has gk = 1.
-  double gr[kMaxK];
-  double gk[kMaxK];
+  // gates, indexed by local bit: U = cos(phi) I - i sin(phi) P with the
+  // half-angle phi reduced to [-pi/2, pi/2] (a global sign of -1 per reduced
+  // gate is dropped: it cancels in every expectation and gradient), stored as
+  // ga = tan(phi / 2), gb = sin(phi) for the three-shear rotation.  Inverse
+  // gates have phi negated.  G_NONE: no gate on that bit.
+  double ga[kMaxK];
+  double gb[kMaxK];
   int32_t gtype[kMaxK];
   int32_t own[kMaxK];    // 1: accumulate the reduction of this local bit into row_base + bit
+  // The same gates per stage slot (stage s, register bit kk -> slot s * R + kk),
+  // filled by the host from the per-bit arrays for the launch's R: a slot whose
+  // bit was handled in an earlier stage is G_NONE.  sflag: SL_ROT_A / SL_ROT_B
+  // (apply to A / B) and SL_RED (reduce this slot's bit).
+  double sa[kMaxSlots];
+  double sb[kMaxSlots];
+  int32_t stype[kMaxSlots];
+  int32_t sflag[kMaxSlots];
+  int32_t slb[kMaxSlots];  // local bit of the slot
   const uint32_t* map;   // ladder map columns (n entries), or null
   const double* init_tab;// n + 1 entries when F_INIT_TAB
   const DTerm* dterms;   // diagonal quadratic form

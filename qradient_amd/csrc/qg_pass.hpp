@@ -34,6 +34,7 @@
 #include <type_traits>
 #include <utility>
 
+#include "qg_gate_asm.hpp"
 #include "qg_internal.h"
 
 namespace qg {
@@ -137,6 +138,18 @@ struct Stor<double> {
   }
 };
 
+// Uniform-index read of a read-only table through the constant address space:
+// compiles to a scalar load (lgkmcnt only), so it never waits on the vector
+// memory counter that tracks the tile's global loads and stores.
+template <class T>
+__device__ __forceinline__ T ldc(const T* p, int i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return ((const __attribute__((address_space(4))) T*)p)[i];
+#else
+  return p[i];  // host pass: the body is parsed, never run
+#endif
+}
+
 // LDS slot of local index j: an XOR swizzle that keeps every stage layout
 // free of bank conflicts (64 banks x 4 B: 32 slots of 8 B or 16 of 16 B per row).
 template <class S>
@@ -171,7 +184,7 @@ __device__ __forceinline__ int gpos_of_local(int l, int c, int p) { return l < c
 
 __device__ __forceinline__ uint32_t apply_map(const uint32_t* __restrict__ m, uint32_t v, int n) {
   uint32_t r = 0;
-  for (int k = 0; k < n; ++k) r ^= m[k] & (0u - ((v >> k) & 1u));
+  for (int k = 0; k < n; ++k) r ^= ldc(m, k) & (0u - ((v >> k) & 1u));
   return r;
 }
 
@@ -226,43 +239,72 @@ __device__ __forceinline__ double recj(const Cd& l, const Cx<T>& p) {  // Re(con
   return l.re * (double)p.re + l.im * (double)p.im;
 }
 
-// sum over the register pairs of bit KK of Im<l|P|p>, in FP64.
+// sum over the register pairs of bit KK of Im<l|P|p>, in FP64: FMA chains
+// into two partial sums (even / odd pairs) added at the end, a fixed order.
+__device__ __forceinline__ double fmd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <int R, int KK, class T>
 __device__ __forceinline__ double red_bit(const Cd (&l)[1 << R], const Cx<T> (&p)[1 << R], int type) {
   constexpr int A = 1 << R;
-  double acc = 0.0;
+  constexpr int M = 1 << KK;
+  double s0 = 0.0, s1 = 0.0;
   if (type == G_Z) {
+    // sum_a sign(a) Im(conj(l_a) p_a)
 #pragma unroll
     for (int a = 0; a < A; ++a) {
-      const double v = imcj(l[a], p[a]);
-      acc += (a & (1 << KK)) ? -v : v;
+      const double pr = (double)p[a].re, pi = (double)p[a].im;
+      double& acc = (a & 1) ? s1 : s0;
+      if (a & M) {
+        acc = fmd(-l[a].re, pi, acc);
+        acc = fmd(l[a].im, pr, acc);
+      } else {
+        acc = fmd(l[a].re, pi, acc);
+        acc = fmd(-l[a].im, pr, acc);
+      }
     }
   } else if (type == G_X) {
+    // Im(conj(l0) p1) + Im(conj(l1) p0)
+    int k = 0;
 #pragma unroll
     for (int a = 0; a < A; ++a) {
-      if (a & (1 << KK)) continue;
-      const int b = a | (1 << KK);
-      acc += imcj(l[a], p[b]) + imcj(l[b], p[a]);
+      if (a & M) continue;
+      const int b = a | M;
+      double& acc = (k++ & 1) ? s1 : s0;
+      acc = fmd(l[a].re, (double)p[b].im, acc);
+      acc = fmd(-l[a].im, (double)p[b].re, acc);
+      acc = fmd(l[b].re, (double)p[a].im, acc);
+      acc = fmd(-l[b].im, (double)p[a].re, acc);
     }
   } else if (type == G_Y) {
+    // Re(conj(l1) p0) - Re(conj(l0) p1)
+    int k = 0;
 #pragma unroll
     for (int a = 0; a < A; ++a) {
-      if (a & (1 << KK)) continue;
-      const int b = a | (1 << KK);
-      acc += recj(l[b], p[a]) - recj(l[a], p[b]);
+      if (a & M) continue;
+      const int b = a | M;
+      double& acc = (k++ & 1) ? s1 : s0;
+      acc = fmd(l[b].re, (double)p[a].re, acc);
+      acc = fmd(l[b].im, (double)p[a].im, acc);
+      acc = fmd(-l[a].re, (double)p[b].re, acc);
+      acc = fmd(-l[a].im, (double)p[b].im, acc);
     }
   }
-  return acc;
+  return s0 + s1;
 }
 
+// The swizzle is GF(2)-linear and the thread and register parts of a local
+// index have disjoint bits, so slot(lidx) = swz(thread part) ^ swz(a << lo):
+// one per-thread base per layout and a uniform term per register.
 template <class S, int R, class T>
 __device__ __forceinline__ void lds_transpose(Cx<T> (&x)[1 << R], void* lds, uint32_t tid, int lo_from, int lo_to) {
   constexpr int A = 1 << R;
+  const uint32_t bw = swz<S>(lidx<R>(tid, 0, lo_from));
+  const uint32_t br = swz<S>(lidx<R>(tid, 0, lo_to));
 #pragma unroll
-  for (int a = 0; a < A; ++a) Stor<S>::lds_st(lds, swz<S>(lidx<R>(tid, a, lo_from)), x[a]);
+  for (int a = 0; a < A; ++a) Stor<S>::lds_st(lds, bw ^ swz<S>((uint32_t)a << lo_from), x[a]);
   __syncthreads();
 #pragma unroll
-  for (int a = 0; a < A; ++a) x[a] = Stor<S>::template lds_ld<T>(lds, swz<S>(lidx<R>(tid, a, lo_to)));
+  for (int a = 0; a < A; ++a) x[a] = Stor<S>::template lds_ld<T>(lds, br ^ swz<S>((uint32_t)a << lo_to));
   __syncthreads();
 }
 
@@ -289,8 +331,9 @@ __device__ __forceinline__ void diag_prepare(const PassDesc* __restrict__ d, uin
   }
   const int nt = d->ndterms;
   for (int i = 0; i < nt; ++i) {
-    const int a = d->dterms[i].a, b = d->dterms[i].b;
-    const double w = d->dterms[i].w;
+    const DTerm dt = ldc(d->dterms, i);
+    const int a = dt.a, b = dt.b;
+    const double w = dt.w;
     int ka = -1, kb = -1;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
@@ -336,11 +379,12 @@ __device__ __forceinline__ double diag_value(const DiagParts<R>& o, int a) {
 }
 
 // exp(-i * gamma * D): PH = 1 reads a table (integer-valued D), PH = 2 evaluates sincos.
+// exp(-i gamma D): PH 1 reads the table staged in LDS (tab), PH 2 computes it.
 template <int PH>
-__device__ __forceinline__ Cd phase_of(const PassDesc* __restrict__ d, double D) {
+__device__ __forceinline__ Cd phase_of(const PassDesc* __restrict__ d, const double* tab, double D) {
   if constexpr (PH == 1) {
     const int idx = (int)rint(D) + d->lut_off;
-    return {d->lut[2 * idx], d->lut[2 * idx + 1]};
+    return {tab[2 * idx], tab[2 * idx + 1]};
   } else {
     double s, c;
     sincos(d->phase_gamma * D, &s, &c);
@@ -376,64 +420,81 @@ __device__ __forceinline__ int nnew_of(const int* los, int s, int R) {
   return d < R ? d : R;
 }
 
-// Factored gate on register bit KK (see PassDesc): applies (I - i r P) for
-// form 0 or (r I - i P) for form 1, one FMA per real output; the factor gk is
-// carried by the caller as a running scale.  code = type + 4 * form.
 __device__ __forceinline__ double fmx(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float fmx(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
+// In-place rotation of the plane (u, v) by phi: [u; v] <- [[cos, -sin], [sin, cos]] [u; v]
+// as three shears with a = tan(phi / 2), b = sin(phi) (|phi| <= pi / 2, so
+// |a|, |b| <= 1): one FMA each, every update in place.
+//
+// The FMAs are written as inline asm with the accumulator tied to the output
+// register: every gate type then updates the same registers, and the join
+// after the per-bit gate-type branch needs no register copies.
+__device__ __forceinline__ void shear3(double& u, double& v, double a, double b) {
+  asm("v_fma_f64 %0, -%1, %2, %0" : "+v"(u) : "s"(a), "v"(v));
+  asm("v_fma_f64 %0, %1, %2, %0" : "+v"(v) : "s"(b), "v"(u));
+  asm("v_fma_f64 %0, -%1, %2, %0" : "+v"(u) : "s"(a), "v"(v));
+}
+__device__ __forceinline__ void shear3(float& u, float& v, float a, float b) {
+  asm("v_fma_f32 %0, -%1, %2, %0" : "+v"(u) : "v"(a), "v"(v));
+  asm("v_fma_f32 %0, %1, %2, %0" : "+v"(v) : "v"(b), "v"(u));
+  asm("v_fma_f32 %0, -%1, %2, %0" : "+v"(u) : "v"(a), "v"(v));
+}
+
+// U = cos(phi) I - i sin(phi) P on register bit KK (see PassDesc), as two
+// plane rotations per amplitude pair: X rotates (re0, im1) by -phi and
+// (im0, re1) by +phi, Y rotates (re0, re1) and (im0, im1) by +phi, Z rotates
+// (re0, im0) by -phi and (re1, im1) by +phi.
+// Amplitude order of the pairs of register bit KK (pair p = entries 2p, 2p+1).
+struct PairOrder {
+  int m[16];
+};
+__host__ __device__ constexpr PairOrder pair_order(int kk) {
+  PairOrder o{};
+  int p = 0;
+  for (int i = 0; i < 16; ++i)
+    if (!(i & (1 << kk))) {
+      o.m[2 * p] = i;
+      o.m[2 * p + 1] = i | (1 << kk);
+      ++p;
+    }
+  return o;
+}
+
 template <int R, int KK, class T>
-__device__ __forceinline__ void gate_bit_f(Cx<T> (&x)[1 << R], int code, double rd) {
+__device__ __forceinline__ void gate_bit_s(Cx<T> (&x)[1 << R], int type, double ad, double bd) {
   constexpr int A = 1 << R;
   constexpr int M = 1 << KK;
-  const T r = (T)rd;
-  if (code == G_X) {
-#pragma unroll
-    for (int a = 0; a < A; ++a) {
-      if (a & M) continue;
-      const Cx<T> x0 = x[a], x1 = x[a | M];
-      x[a] = {fmx(r, x1.im, x0.re), fmx(-r, x1.re, x0.im)};
-      x[a | M] = {fmx(r, x0.im, x1.re), fmx(-r, x0.re, x1.im)};
+  if constexpr (R == 4) {
+    constexpr PairOrder po = pair_order(KK);
+    if constexpr (std::is_same<T, double>::value) {
+      asm(QG_GATE_ASM("v_fma_f64") : QG_GATE_OPERANDS(x, po.m) : "s"(type), "s"(ad), "s"(bd) : "scc");
+    } else {
+      const float a = (float)ad, b = (float)bd;
+      asm(QG_GATE_ASM("v_fma_f32") : QG_GATE_OPERANDS(x, po.m) : "s"(type), "v"(a), "v"(b) : "scc");
     }
-  } else if (code == G_X + 4) {
+    return;
+  }
+  const T a = (T)ad, b = (T)bd, na = (T)(-ad), nb = (T)(-bd);
+  if (type == G_X) {
 #pragma unroll
-    for (int a = 0; a < A; ++a) {
-      if (a & M) continue;
-      const Cx<T> x0 = x[a], x1 = x[a | M];
-      x[a] = {fmx(r, x0.re, x1.im), fmx(r, x0.im, -x1.re)};
-      x[a | M] = {fmx(r, x1.re, x0.im), fmx(r, x1.im, -x0.re)};
+    for (int i = 0; i < A; ++i) {
+      if (i & M) continue;
+      shear3(x[i].re, x[i | M].im, na, nb);
+      shear3(x[i].im, x[i | M].re, a, b);
     }
-  } else if (code == G_Y) {
+  } else if (type == G_Y) {
 #pragma unroll
-    for (int a = 0; a < A; ++a) {
-      if (a & M) continue;
-      const Cx<T> x0 = x[a], x1 = x[a | M];
-      x[a] = {fmx(-r, x1.re, x0.re), fmx(-r, x1.im, x0.im)};
-      x[a | M] = {fmx(r, x0.re, x1.re), fmx(r, x0.im, x1.im)};
+    for (int i = 0; i < A; ++i) {
+      if (i & M) continue;
+      shear3(x[i].re, x[i | M].re, a, b);
+      shear3(x[i].im, x[i | M].im, a, b);
     }
-  } else if (code == G_Y + 4) {
+  } else if (type == G_Z) {
 #pragma unroll
-    for (int a = 0; a < A; ++a) {
-      if (a & M) continue;
-      const Cx<T> x0 = x[a], x1 = x[a | M];
-      x[a] = {fmx(r, x0.re, -x1.re), fmx(r, x0.im, -x1.im)};
-      x[a | M] = {fmx(r, x1.re, x0.re), fmx(r, x1.im, x0.im)};
-    }
-  } else if (code == G_Z) {
-#pragma unroll
-    for (int a = 0; a < A; ++a) {
-      const Cx<T> v = x[a];
-      // bit 0: (1 - i r) v ; bit 1: (1 + i r) v
-      const T rr = (a & M) ? -r : r;
-      x[a] = {fmx(rr, v.im, v.re), fmx(-rr, v.re, v.im)};
-    }
-  } else if (code == G_Z + 4) {
-#pragma unroll
-    for (int a = 0; a < A; ++a) {
-      const Cx<T> v = x[a];
-      // bit 0: (r - i) v ; bit 1: (r + i) v
-      if (a & M) x[a] = {fmx(r, v.re, -v.im), fmx(r, v.im, v.re)};
-      else x[a] = {fmx(r, v.re, v.im), fmx(r, v.im, -v.re)};
+    for (int i = 0; i < A; ++i) {
+      if (i & M) shear3(x[i].re, x[i].im, a, b);
+      else shear3(x[i].re, x[i].im, na, nb);
     }
   }
 }
@@ -451,7 +512,7 @@ __device__ __forceinline__ void tile_addresses(const PassDesc* __restrict__ d, b
       uint32_t v = mbase;
 #pragma unroll
       for (int k = 0; k < R; ++k)
-        if (a & (1 << k)) v ^= d->map[rpos[k]];
+        if (a & (1 << k)) v ^= ldc(d->map, rpos[k]);
       adr[a] = v;
     }
   } else {
@@ -472,7 +533,9 @@ __device__ __forceinline__ void tile_addresses(const PassDesc* __restrict__ d, b
 //   R  : register bits per thread (2^R amplitudes)
 //   TWO: an adjoint pass with the companion state A and the propagated B
 //   PH : diagonal phase method (0 none, 1 table, 2 sincos)
-//   TA : register precision of A in a TWO pass (the propagated state is FP64)
+//   POST: the diagonal ops after the gates (phase, d/dgamma reduction,
+//        observable) are compiled in; without them the adjoint pass needs
+//        fewer registers
 //
 // The next tile's loads are issued before the current tile is processed (the
 // raw storage-type values wait in registers), except in complex128 adjoint
@@ -482,13 +545,13 @@ __device__ __forceinline__ void tile_addresses(const PassDesc* __restrict__ d, b
 // end; complex128 adjoint passes reduce per tile into per-wave slots.  Both
 // use a fixed order, so results are bitwise reproducible.
 // ---------------------------------------------------------------------------
-template <class S, int R, bool TWO, int PH, class TA, bool BIG>
+template <class S, int R, bool TWO, int PH, bool POST, bool BIG>
 __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* lds, uint32_t tid_in, uint32_t blk,
                                          uint32_t nblk) {
   constexpr int A = 1 << R;
   constexpr bool PF = !(TWO && sizeof(S) == 8);
   constexpr bool ACCL = TWO && sizeof(S) == 4;
-  typedef typename std::conditional<TWO, TA, double>::type TAr;
+  typedef double TAr;
   const int n = d->n, K = d->K, t = d->t, c = d->c, p = d->p;
   const uint32_t flags = d->flags;
   const uint32_t ntiles = 1u << (n - K);
@@ -518,19 +581,30 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
   }
 
   // LDS after the tile buffer: per-wave FP64 slots [wave][K + 1] (lane 0 of
-  // each wave is the only writer of its row), then for ACCL the per-thread
-  // accumulators [local bit][thread].
+  // each wave is the only writer of its row), for ACCL the per-thread
+  // accumulators [local bit][thread], then the staged table (the initial
+  // state's popcount table or the phase table).
   double* wacc = reinterpret_cast<double*>(reinterpret_cast<char*>(lds) + (sizeof(S) * 2u << K));
   double* tacc = wacc + nwaves * (kMaxK + 1);
+  double* tab = tacc + (ACCL ? K * (int)nthr : 0);
   const bool reduce = TWO && (flags & (F_REDUCE | F_DRED_POST | F_OBS_DIAG)) && d->row_base >= 0;
   const bool reduce_bits = reduce && (flags & F_REDUCE);
   const uint32_t wave = tid_in >> 6;
   const bool lane0 = (tid_in & 63u) == 0u;
-  if (reduce) {
-    for (uint32_t i = tid_in; i < (uint32_t)(nwaves * (K + 1)); i += nthr) wacc[i] = 0.0;
-    if constexpr (ACCL) {
-      for (int lb = 0; lb < K; ++lb) tacc[lb * nthr + tid_in] = 0.0;
+  // staged tables: the phase table (PH 1) first, then the popcount table
+  const bool use_lut = PH == 1 && (flags & (F_PHASE_PRE | F_PHASE_POST));
+  const int nlut = use_lut ? 2 * (2 * d->lut_off + 1) : 0;
+  const int ninit = (flags & F_INIT_TAB) ? n + 1 : 0;
+  double* itab = tab + nlut;
+  if (reduce || nlut + ninit > 0) {
+    if (reduce) {
+      for (uint32_t i = tid_in; i < (uint32_t)(nwaves * (K + 1)); i += nthr) wacc[i] = 0.0;
+      if constexpr (ACCL) {
+        for (int lb = 0; lb < K; ++lb) tacc[lb * nthr + tid_in] = 0.0;
+      }
     }
+    for (int i = (int)tid_in; i < nlut; i += (int)nthr) tab[i] = d->lut[i];
+    for (int i = (int)tid_in; i < ninit; i += (int)nthr) itab[i] = d->init_tab[i];
     __syncthreads();
   }
   double dacc = 0.0;
@@ -589,13 +663,12 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
       uint32_t adr[A];
       tile_addresses<R>(d, gather, gthr, rpos, rbit, n, adr);
 #pragma unroll
-      for (int a = 0; a < A; ++a) xa[a] = {(TAr)d->init_tab[__popc(adr[a])], (TAr)0};
+      for (int a = 0; a < A; ++a) xa[a] = {(TAr)itab[__popc(adr[a])], (TAr)0};
     }
     if constexpr (TWO) {
 #pragma unroll
       for (int a = 0; a < A; ++a) xb[a] = load_b ? Cd{(double)rb[a].re, (double)rb[a].im} : Cd{0.0, 0.0};
     }
-    if (PF && r + 1 < rounds) fetch(tile_of(r + 1));
 
     // ---- diagonal ops in the natural layout, before the gates --------------
     if (flags & F_CZ_PRE) {
@@ -614,13 +687,14 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
         DiagParts<R> dp;
         diag_prepare<R>(d, gthr, rpos, dp);
 #pragma unroll
-        for (int a = 0; a < A; ++a) xa[a] = cmul(phase_of<PH>(d, diag_value<R>(dp, a)), xa[a]);
+        for (int a = 0; a < A; ++a) xa[a] = cmul(phase_of<PH>(d, tab, diag_value<R>(dp, a)), xa[a]);
       }
     }
+    // the next tile's loads go out after everything that reads this tile's
+    // inputs, so no wait for this tile's data covers them
+    if (PF && r + 1 < rounds) fetch(tile_of(r + 1));
 
     // ---- stages: reductions then gates on each newly visible register bit --
-    // kA / kB: running products of the gate factors applied to A / B
-    double kA = 1.0, kB = 1.0;
 #pragma unroll 1
     for (int s = 0; s < kMaxStages; ++s) {
       if (s < ns) {
@@ -628,40 +702,31 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
           lds_transpose<S, R>(xa, lds, tid, los[s - 1], los[s]);
           if constexpr (TWO) lds_transpose<S, R>(xb, lds, tid, los[s - 1], los[s]);
         }
-        const int lo = los[s];
-        const int nnew = nnew_of(los, s, R);
-        const bool last_stage = (s == ns - 1);
         static_for<R>([&](auto kc) __attribute__((always_inline)) {
           constexpr int kk = decltype(kc)::value;
-          if (kk < nnew) {
-            // opaque to the optimiser: keeps the uniform gate loads next to
-            // their use instead of hoisting them out of the loops
-            int lb = __builtin_amdgcn_readfirstlane(lo + kk);
-            asm volatile("" : "+s"(lb));
-            const int code = d->gtype[lb];
-            if constexpr (TWO) {
-              if (reduce_bits && d->own[lb]) {
-                const double v = red_bit<R, kk>(xb, xa, code & 3) * (kA * kB);
-                if constexpr (ACCL) {
-                  tacc[lb * nthr + tid] += v;
-                } else {
-                  const double w = wave_sum(v, width);
-                  if ((tid & 63) == 0) wacc[(tid >> 6) * (K + 1) + lb] += w;
-                }
+          // opaque to the optimiser: keeps the uniform slot loads next to
+          // their use instead of hoisting them out of the loops
+          int sl = __builtin_amdgcn_readfirstlane(s * R + kk);
+          asm volatile("" : "+s"(sl));
+          const int ty = ldc(d->stype, sl), fl = ldc(d->sflag, sl);
+          if constexpr (TWO) {
+            if (fl & SL_RED) {
+              const int lb = ldc(d->slb, sl);
+              const double v = red_bit<R, kk>(xb, xa, ty);
+              if constexpr (ACCL) {
+                tacc[lb * nthr + tid] += v;
+              } else {
+                const double w = wave_sum(v, width);
+                if ((tid & 63) == 0) wacc[(tid >> 6) * (K + 1) + lb] += w;
               }
             }
-            const double gr = d->gr[lb], gk = d->gk[lb];
-            const bool need_a = a_used_after || !last_stage || kk < nnew - 1;
-            if ((flags & F_ROT_A) && need_a) {
-              gate_bit_f<R, kk>(xa, code, gr);
-              kA *= gk;
-            }
-            if constexpr (TWO) {
-              if (flags & F_ROT_B) {
-                gate_bit_f<R, kk>(xb, code, gr);
-                kB *= gk;
-              }
-            }
+          }
+          const double ga = ldc(d->sa, sl), gb = ldc(d->sb, sl);
+          if constexpr (TWO) {
+            if (fl & SL_ROT_A) gate_bit_s<R, kk>(xa, ty, ga, gb);
+            if (fl & SL_ROT_B) gate_bit_s<R, kk>(xb, ty, ga, gb);
+          } else {
+            gate_bit_s<R, kk>(xa, ty, ga, gb);
           }
         });
       }
@@ -670,21 +735,9 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
       lds_transpose<S, R>(xa, lds, tid, los[ns - 1], t);
       if constexpr (TWO) lds_transpose<S, R>(xb, lds, tid, los[ns - 1], t);
     }
-    // restore the gate factors
-    if (a_used_after && kA != 1.0) {
-      const TAr ka = (TAr)kA;
-#pragma unroll
-      for (int a = 0; a < A; ++a) xa[a] = {ka * xa[a].re, ka * xa[a].im};
-    }
-    if constexpr (TWO) {
-      if (kB != 1.0) {
-#pragma unroll
-        for (int a = 0; a < A; ++a) xb[a] = {kB * xb[a].re, kB * xb[a].im};
-      }
-    }
 
     // ---- diagonal ops in the natural layout, after the gates --------------
-    if (flags & (F_DRED_POST | F_PHASE_POST | F_OBS_DIAG)) {
+    if (POST && (flags & (F_DRED_POST | F_PHASE_POST | F_OBS_DIAG))) {
       DiagParts<R> dp;
       diag_prepare<R>(d, gthr, rpos, dp);
       double xacc = 0.0;
@@ -696,7 +749,7 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
         }
         if constexpr (PH != 0) {
           if (flags & F_PHASE_POST) {
-            const Cd ph = phase_of<PH>(d, D);
+            const Cd ph = phase_of<PH>(d, tab, D);
             xa[a] = cmul(ph, xa[a]);
             if constexpr (TWO) xb[a] = cmul(ph, xb[a]);
           }
@@ -757,7 +810,7 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
       }
     }
   }
-  if (flags & (F_DRED_POST | F_OBS_DIAG)) {
+  if (POST && (flags & (F_DRED_POST | F_OBS_DIAG))) {
     const double v = wave_sum(dacc, width);
     if (lane0) wacc[wave * (K + 1) + K] = v;
   }
@@ -779,45 +832,46 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
 
 // blockIdx.y selects one of gridDim.y consecutive descriptors: a batched
 // launch runs the same pass of gridDim.y independent circuits.
-template <class S, int R, bool TWO, int PH, class TA, bool BIG>
+template <class S, int R, bool TWO, int PH, bool POST, bool BIG>
 __global__ __launch_bounds__(512) void k_pass(const PassDesc* __restrict__ d) {
   extern __shared__ __align__(16) char lds_dyn[];
-  run_pass<S, R, TWO, PH, TA, BIG>(d + blockIdx.y, lds_dyn, threadIdx.x, blockIdx.x, gridDim.x);
+  run_pass<S, R, TWO, PH, POST, BIG>(d + blockIdx.y, lds_dyn, threadIdx.x, blockIdx.x, gridDim.x);
 }
 
 // LDS bytes of a pass: the tile in the storage type, the per-wave slots and,
 // for complex64 adjoint passes, the per-thread accumulators.
-__host__ __device__ inline size_t pass_lds_bytes(int dtype, int K, int threads, bool two) {
+// tab_bytes: the staged table of the pass (0 if none).
+__host__ __device__ inline size_t pass_lds_bytes(int dtype, int K, int threads, bool two, int tab_bytes = 0) {
   return (size_t)(1u << K) * (dtype == 1 ? 8 : 16) + (size_t)((threads + 63) / 64) * (kMaxK + 1) * sizeof(double) +
-         (two && dtype == 1 ? (size_t)K * threads * sizeof(double) : 0);
+         (two && dtype == 1 ? (size_t)K * threads * sizeof(double) : 0) + (size_t)tab_bytes;
 }
 
 // BIG: the state is 4 GiB or larger, so byte offsets need 64-bit addresses.
 // With occ != NULL nothing is launched: *occ receives the number of
 // workgroups of this instantiation that fit one CU.
-template <class S, int R, bool TWO, int PH, class TA>
+template <class S, int R, bool TWO, int PH, bool POST>
 hipError_t launch_pass_t(const PassDesc* d, int K, int grid, int batch, int threads, bool big, hipStream_t st,
-                         int* occ) {
-  const size_t lds = pass_lds_bytes(sizeof(S) == 4 ? 1 : 0, K, threads, TWO);
+                         int* occ, int tab_bytes) {
+  const size_t lds = pass_lds_bytes(sizeof(S) == 4 ? 1 : 0, K, threads, TWO, tab_bytes);
   if (occ) {
     if (big)
-      return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_pass<S, R, TWO, PH, TA, true>, threads, lds);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_pass<S, R, TWO, PH, TA, false>, threads, lds);
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_pass<S, R, TWO, PH, POST, true>, threads, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_pass<S, R, TWO, PH, POST, false>, threads, lds);
   }
   if (big)
-    hipLaunchKernelGGL((k_pass<S, R, TWO, PH, TA, true>), dim3(grid, batch), dim3(threads), lds, st, d);
+    hipLaunchKernelGGL((k_pass<S, R, TWO, PH, POST, true>), dim3(grid, batch), dim3(threads), lds, st, d);
   else
-    hipLaunchKernelGGL((k_pass<S, R, TWO, PH, TA, false>), dim3(grid, batch), dim3(threads), lds, st, d);
+    hipLaunchKernelGGL((k_pass<S, R, TWO, PH, POST, false>), dim3(grid, batch), dim3(threads), lds, st, d);
   return hipGetLastError();
 }
 
-template <class S, int R, bool TWO, int PH, class TA>
+template <class S, int R, bool TWO, int PH, bool POST>
 hipError_t set_attr_t() {
-  const int maxlds = (int)pass_lds_bytes(sizeof(S) == 4 ? 1 : 0, kMaxK, 512, TWO);
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pass<S, R, TWO, PH, TA, false>),
+  const int maxlds = (int)pass_lds_bytes(sizeof(S) == 4 ? 1 : 0, kMaxK, 512, TWO, 16 * kLutMax + 8 * (kMaxQubits + 1));
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pass<S, R, TWO, PH, POST, false>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, maxlds);
   if (e == hipSuccess)
-    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pass<S, R, TWO, PH, TA, true>),
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pass<S, R, TWO, PH, POST, true>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, maxlds);
   return e;
 }

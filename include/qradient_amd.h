@@ -132,6 +132,8 @@ int qg_plan_tune(qg_plan* plan, int kmax, int cmin, int r_fwd, int r_rev);
 /* on = 1 (default): circuits with n <= kmax run whole in one workgroup each
  * (one launch per call, FP64 throughout); on = 0 uses the pass kernels. */
 int qg_plan_set_fused(qg_plan* plan, int on);
+/* The checkpoint period a McClean plan chose (0: full per-pass history). */
+int qg_plan_ckpt_every(qg_plan* plan, int* out);
 int qg_plan_destroy(qg_plan* plan);
 
 /* ---- device-resident State (qradient/physical_components/state.py) --------

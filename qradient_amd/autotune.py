@@ -56,7 +56,7 @@ def cached(kind, n, L, dtype, entangler):
 def candidates(dtype):
     out = []
     for kmax in (13, 12, 11):
-        for cmin in (2, 3, 4):
+        for cmin in (2, 3, 4, 5, 6):
             rs = (4,)
             for r_fwd in rs:
                 for r_rev in rs:

@@ -547,6 +547,11 @@ static void desc_slots(PassDesc& d, int R, bool two) {
   }
   // A's last gate matters only if A is used after the gates
   if (!a_used_after && last_a >= 0) d.sflag[last_a] &= ~SL_ROT_A;
+  // stages after the last one with work (only spectator bits) are skipped
+  d.nstage = 1;
+  for (int s = 0; s < ns; ++s)
+    for (int kk = 0; kk < R; ++kk)
+      if (d.stype[s * R + kk] != G_NONE || (d.sflag[s * R + kk] & SL_RED)) d.nstage = s + 1;
 }
 
 static int run_launches(qg_plan* p, int ndesc) {
@@ -555,6 +560,8 @@ static int run_launches(qg_plan* p, int ndesc) {
     if (l.kind == 0) {
       for (int b = 0; b < l.batch; ++b) desc_slots(p->h_desc[l.desc + b], l.R, l.two);
       const PassDesc& d = p->h_desc[l.desc];
+      // passes without a phase use the kernel variant without phase code
+      if (!(d.flags & (F_PHASE_PRE | F_PHASE_POST))) l.phase = 0;
       l.post = l.two && (d.flags & (F_DRED_POST | F_PHASE_POST | F_OBS_DIAG)) != 0;
       l.tab = ((d.flags & F_INIT_TAB) ? (int)sizeof(double) * (p->n + 1) : 0) +
               ((l.phase == 1 && (d.flags & (F_PHASE_PRE | F_PHASE_POST))) ? (int)sizeof(double) * 2 * (2 * d.lut_off + 1)
@@ -1291,6 +1298,9 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
       l.post = tf;
       l.threads = G.threads;
       l.grid = grid_for_pass(p, G, two, tf, phase);
+      // a batched launch fills the device across circuits: fewer workgroups
+      // per circuit, each running more tiles (amortises its prologue)
+      while (l.grid > 1 && (uint64_t)l.grid * nb > 2ull * (uint64_t)p->ctx->num_cu) l.grid >>= 1;
       l.phase = phase;
       l.two = two;
       l.kind = 0;
@@ -1679,5 +1689,11 @@ extern "C" int qg_plan_tune(qg_plan* p, int kmax, int cmin, int r_fwd, int r_rev
 extern "C" int qg_plan_set_fused(qg_plan* p, int on) {
   if (!p) return fail(QG_ERR_INVALID, "NULL plan");
   p->fused = on != 0;
+  return QG_OK;
+}
+
+extern "C" int qg_plan_ckpt_every(qg_plan* p, int* out) {
+  if (!p || !out) return fail(QG_ERR_INVALID, "NULL argument");
+  *out = p->history ? 0 : p->ckpt;
   return QG_OK;
 }

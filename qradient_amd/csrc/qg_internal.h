@@ -75,6 +75,7 @@ struct PassDesc {
   int32_t stype[kMaxSlots];
   int32_t sflag[kMaxSlots];
   int32_t slb[kMaxSlots];  // local bit of the slot
+  int32_t nstage;          // stages that hold a gate or a reduction (later stages are skipped)
   const uint32_t* map;   // ladder map columns (n entries), or null
   const double* init_tab;// n + 1 entries when F_INIT_TAB
   const DTerm* dterms;   // diagonal quadratic form

@@ -96,6 +96,19 @@ struct Stor<float> {
     const f2v v = ((const lf2v*)lds)[i];
     return {(T)v.x, (T)v.y};
   }
+  // byte-offset forms
+  template <class T>
+  static __device__ __forceinline__ void lds_st_b(void* lds, uint32_t off, Cx<T> v) {
+    f2v w;
+    w.x = (float)v.re;
+    w.y = (float)v.im;
+    *(lf2v*)((char*)lds + off) = w;
+  }
+  template <class T>
+  static __device__ __forceinline__ Cx<T> lds_ld_b(const void* lds, uint32_t off) {
+    const f2v v = *(const lf2v*)((const char*)lds + off);
+    return {(T)v.x, (T)v.y};
+  }
 };
 template <>
 struct Stor<double> {
@@ -136,6 +149,18 @@ struct Stor<double> {
     const d2v v = ((const ld2v*)lds)[i];
     return {(T)v.x, (T)v.y};
   }
+  template <class T>
+  static __device__ __forceinline__ void lds_st_b(void* lds, uint32_t off, Cx<T> v) {
+    d2v w;
+    w.x = (double)v.re;
+    w.y = (double)v.im;
+    *(ld2v*)((char*)lds + off) = w;
+  }
+  template <class T>
+  static __device__ __forceinline__ Cx<T> lds_ld_b(const void* lds, uint32_t off) {
+    const d2v v = *(const ld2v*)((const char*)lds + off);
+    return {(T)v.x, (T)v.y};
+  }
 };
 
 // Uniform-index read of a read-only table through the constant address space:
@@ -150,11 +175,12 @@ __device__ __forceinline__ T ldc(const T* p, int i) {
 #endif
 }
 
-// LDS slot of local index j: an XOR swizzle that keeps every stage layout
-// free of bank conflicts (64 banks x 4 B: 32 slots of 8 B or 16 of 16 B per row).
+// LDS slot of local index j: a GF(2)-linear XOR swizzle (a bijection on
+// [0, 2^K)) under which every stage layout's 8-byte (complex64) or 16-byte
+// (complex128) accesses are free of bank conflicts, for 32 or 64 banks of 4 B.
 template <class S>
 __device__ __forceinline__ uint32_t swz(uint32_t j) {
-  if constexpr (sizeof(S) == 4) return j ^ ((j >> 5) & 31u);
+  if constexpr (sizeof(S) == 4) return j ^ ((j >> 4) & 31u);
   else return j ^ ((j >> 4) & 15u);
 }
 
@@ -295,16 +321,19 @@ __device__ __forceinline__ double red_bit(const Cd (&l)[1 << R], const Cx<T> (&p
 // The swizzle is GF(2)-linear and the thread and register parts of a local
 // index have disjoint bits, so slot(lidx) = swz(thread part) ^ swz(a << lo):
 // one per-thread base per layout and a uniform term per register.
+// Byte offsets are formed directly (the slot shift is linear too), so each
+// access costs one VALU XOR with a uniform term.
 template <class S, int R, class T>
 __device__ __forceinline__ void lds_transpose(Cx<T> (&x)[1 << R], void* lds, uint32_t tid, int lo_from, int lo_to) {
   constexpr int A = 1 << R;
-  const uint32_t bw = swz<S>(lidx<R>(tid, 0, lo_from));
-  const uint32_t br = swz<S>(lidx<R>(tid, 0, lo_to));
+  constexpr int SH = sizeof(S) == 4 ? 3 : 4;
+  const uint32_t bw = swz<S>(lidx<R>(tid, 0, lo_from)) << SH;
+  const uint32_t br = swz<S>(lidx<R>(tid, 0, lo_to)) << SH;
 #pragma unroll
-  for (int a = 0; a < A; ++a) Stor<S>::lds_st(lds, bw ^ swz<S>((uint32_t)a << lo_from), x[a]);
+  for (int a = 0; a < A; ++a) Stor<S>::lds_st_b(lds, bw ^ (swz<S>((uint32_t)a << lo_from) << SH), x[a]);
   __syncthreads();
 #pragma unroll
-  for (int a = 0; a < A; ++a) x[a] = Stor<S>::template lds_ld<T>(lds, br ^ swz<S>((uint32_t)a << lo_to));
+  for (int a = 0; a < A; ++a) x[a] = Stor<S>::template lds_ld_b<T>(lds, br ^ (swz<S>((uint32_t)a << lo_to) << SH));
   __syncthreads();
 }
 
@@ -539,7 +568,8 @@ __device__ __forceinline__ void tile_addresses(const PassDesc* __restrict__ d, b
 //
 // The next tile's loads are issued before the current tile is processed (the
 // raw storage-type values wait in registers), except in complex128 adjoint
-// passes where that would double the register footprint.  In complex64
+// passes and adjoint passes with post-gate diagonal ops, where the registers
+// are needed for the states and the diagonal terms.  In complex64
 // adjoint passes the reductions are accumulated per thread across all tiles in
 // LDS slots [local bit][thread] and reduced across the workgroup once, at the
 // end; complex128 adjoint passes reduce per tile into per-wave slots.  Both
@@ -549,7 +579,7 @@ template <class S, int R, bool TWO, int PH, bool POST, bool BIG>
 __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* lds, uint32_t tid_in, uint32_t blk,
                                          uint32_t nblk) {
   constexpr int A = 1 << R;
-  constexpr bool PF = !(TWO && sizeof(S) == 8);
+  constexpr bool PF = !(TWO && (sizeof(S) == 8 || POST));
   constexpr bool ACCL = TWO && sizeof(S) == 4;
   typedef double TAr;
   const int n = d->n, K = d->K, t = d->t, c = d->c, p = d->p;
@@ -571,6 +601,8 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
       ns = s + 1;
     }
   }
+  // trailing stages that only hold spectator bits are skipped
+  if (d->nstage < ns) ns = d->nstage > 0 ? d->nstage : 1;
 
   int rpos[R];
   uint32_t rbit[R];
@@ -687,7 +719,10 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
         DiagParts<R> dp;
         diag_prepare<R>(d, gthr, rpos, dp);
 #pragma unroll
-        for (int a = 0; a < A; ++a) xa[a] = cmul(phase_of<PH>(d, tab, diag_value<R>(dp, a)), xa[a]);
+        for (int a = 0; a < A; ++a) {
+          xa[a] = cmul(phase_of<PH>(d, tab, diag_value<R>(dp, a)), xa[a]);
+          __builtin_amdgcn_sched_barrier(0);
+        }
       }
     }
     // the next tile's loads go out after everything that reads this tile's
@@ -761,6 +796,9 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
             xacc += D * (ar * ar + ai * ai);
           }
         }
+        // one amplitude at a time: keeps the scheduler from evaluating all
+        // sixteen diagonal values and phases ahead (register pressure)
+        __builtin_amdgcn_sched_barrier(0);
       }
       dacc += xacc;
     }

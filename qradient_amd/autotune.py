@@ -17,7 +17,7 @@ import numpy as np
 
 from . import _native
 
-DEFAULT = {'kmax': 13, 'cmin': 2, 'r_fwd': 4, 'r_rev': 4}
+DEFAULT = {'kmax': 13, 'cmin': 3, 'r_fwd': 4, 'r_rev': 4}
 
 
 def cache_path():

@@ -286,7 +286,7 @@ struct qg_plan {
   int ckpt_req = 0;  // requested period (0: history when it fits)
   int nckpt = 0;
   size_t off_work = 0, off_ckpt = 0;
-  int kmax = kMaxK, cmin = 2;
+  int kmax = kMaxK, cmin = 3;  // cmin 3: contiguous runs of >= 8 amplitudes in high-bit passes
   int r_fwd = 4, r_rev = 4;  // amplitudes per thread (log2) of forward / history-adjoint passes (complex64)
   int chunk = 1;  // circuits resident at once (QAOA batches)
   bool fused = true;       // n <= kmax: whole circuit per workgroup (qg_fused.hip)
@@ -506,7 +506,9 @@ static bool set_gate(PassDesc& d, int lb, int axis, double angle, double sign) {
 // arrays, for a launch holding 2^R amplitudes per thread (the kernel's stage
 // layout: stage 0 holds local bits [t, t + R), stage s >= 1 holds
 // [max(0, t - R s), +R); a bit is gated in the first stage that holds it).
-static void desc_slots(PassDesc& d, int R, bool two) {
+// red_first: the adjoint pass takes its reductions before the gates (A is not
+// rotated; see the kernel's POST parameter).
+static void desc_slots(PassDesc& d, int R, bool two, bool red_first) {
   int los[kMaxStages];
   int ns = 1;
   los[0] = d.t;
@@ -535,13 +537,13 @@ static void desc_slots(PassDesc& d, int R, bool two) {
       d.sb[slot] = d.gb[lb];
       int f = 0;
       if (d.gtype[lb] != G_NONE) {
-        if (d.flags & F_ROT_A) {
+        if ((d.flags & F_ROT_A) && !red_first) {
           f |= SL_ROT_A;
           last_a = slot;
         }
         if (two && (d.flags & F_ROT_B)) f |= SL_ROT_B;
       }
-      if (two && (d.flags & F_REDUCE) && d.own[lb] && d.row_base >= 0) f |= SL_RED;
+      if (two && !red_first && (d.flags & F_REDUCE) && d.own[lb] && d.row_base >= 0) f |= SL_RED;
       d.sflag[slot] = f;
     }
   }
@@ -558,11 +560,14 @@ static int run_launches(qg_plan* p, int ndesc) {
   qg_ctx* c = p->ctx;
   for (Launch& l : p->launches)
     if (l.kind == 0) {
-      for (int b = 0; b < l.batch; ++b) desc_slots(p->h_desc[l.desc + b], l.R, l.two);
       const PassDesc& d = p->h_desc[l.desc];
       // passes without a phase use the kernel variant without phase code
       if (!(d.flags & (F_PHASE_PRE | F_PHASE_POST))) l.phase = 0;
-      l.post = l.two && (d.flags & (F_DRED_POST | F_PHASE_POST | F_OBS_DIAG)) != 0;
+      // adjoint passes whose A is a read-only pass output take the
+      // reduce-first form (POST = false); the rest the general form
+      l.post = l.two && ((d.flags & (F_DRED_POST | F_PHASE_POST | F_OBS_DIAG | F_STORE_A | F_PHASE_PRE)) != 0 ||
+                         !(d.flags & F_LOAD_A) || l.phase != 0);
+      for (int b = 0; b < l.batch; ++b) desc_slots(p->h_desc[l.desc + b], l.R, l.two, l.two && !l.post);
       l.tab = ((d.flags & F_INIT_TAB) ? (int)sizeof(double) * (p->n + 1) : 0) +
               ((l.phase == 1 && (d.flags & (F_PHASE_PRE | F_PHASE_POST))) ? (int)sizeof(double) * 2 * (2 * d.lut_off + 1)
                                                                            : 0);

@@ -562,9 +562,14 @@ __device__ __forceinline__ void tile_addresses(const PassDesc* __restrict__ d, b
 //   R  : register bits per thread (2^R amplitudes)
 //   TWO: an adjoint pass with the companion state A and the propagated B
 //   PH : diagonal phase method (0 none, 1 table, 2 sincos)
-//   POST: the diagonal ops after the gates (phase, d/dgamma reduction,
-//        observable) are compiled in; without them the adjoint pass needs
-//        fewer registers
+//   POST: (adjoint passes) the general two-state form, with the diagonal ops
+//        after the gates (phase, d/dgamma reduction, observable) compiled in:
+//        A and B are both rotated and every reduction is taken in the stage
+//        that holds its bit.  Without POST an adjoint pass takes the
+//        reduce-first form: A is the pass output psi (read-only), every
+//        reduction Im<B|P_q|A> is taken in the natural layout before any gate
+//        is undone (each P_q commutes with every gate of the pass, so the pass
+//        output is a valid frame for all of them), and only B is rotated.
 //
 // The next tile's loads are issued before the current tile is processed (the
 // raw storage-type values wait in registers), except in complex128 adjoint
@@ -725,6 +730,75 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
         }
       }
     }
+    // ---- reduce-first adjoint: all reductions in the natural layout -------
+    if constexpr (TWO && !POST) {
+      if (reduce_bits) {
+        // register bits: amplitude pairs inside the thread
+        static_for<R>([&](auto kc) __attribute__((always_inline)) {
+          constexpr int kk = decltype(kc)::value;
+          int lb = __builtin_amdgcn_readfirstlane(t + kk);
+          asm volatile("" : "+s"(lb));
+          if (ldc(d->own, lb)) {
+            const double v = red_bit<R, kk>(xb, xa, ldc(d->gtype, lb));
+            if constexpr (ACCL) {
+              tacc[lb * nthr + tid] += v;
+            } else {
+              const double w = wave_sum(v, width);
+              if ((tid & 63) == 0) wacc[(tid >> 6) * (K + 1) + lb] += w;
+            }
+          }
+        });
+        // thread bits: Z needs only the thread's sum of Im(conj(B) A); X and
+        // Y read the partner amplitude of A from LDS (natural layout)
+        double m0 = 0.0, m1 = 0.0;
+#pragma unroll
+        for (int a = 0; a < A; ++a) {
+          double& m = (a & 1) ? m1 : m0;
+          m = fmd(xb[a].re, xa[a].im, m);
+          m = fmd(-xb[a].im, xa[a].re, m);
+        }
+        const double msum = m0 + m1;
+        constexpr int SH = sizeof(S) == 4 ? 3 : 4;
+#pragma unroll
+        for (int a = 0; a < A; ++a)
+          Stor<S>::lds_st_b(lds, swz<S>(tid | ((uint32_t)a << t)) << SH, xa[a]);
+        __syncthreads();
+#pragma unroll 1
+        for (int lb = 0; lb < t; ++lb) {
+          if (!ldc(d->own, lb)) continue;
+          const int ty = ldc(d->gtype, lb);
+          const double sg = ((tid >> lb) & 1u) ? -1.0 : 1.0;  // +1 where the bit is 0
+          double v;
+          if (ty == G_Z) {
+            v = sg * msum;
+          } else {
+            const uint32_t pb = swz<S>(tid ^ (1u << lb)) << SH;
+            double v0 = 0.0, v1 = 0.0;
+#pragma unroll
+            for (int a = 0; a < A; ++a) {
+              const Cd pa = Stor<S>::template lds_ld_b<double>(lds, pb ^ (swz<S>((uint32_t)a << t) << SH));
+              double& acc = (a & 1) ? v1 : v0;
+              if (ty == G_X) {  // Im(conj(B_x) A_{x^q})
+                acc = fmd(xb[a].re, pa.im, acc);
+                acc = fmd(-xb[a].im, pa.re, acc);
+              } else {          // -sg * Re(conj(B_x) A_{x^q})
+                acc = fmd(xb[a].re, pa.re, acc);
+                acc = fmd(xb[a].im, pa.im, acc);
+              }
+            }
+            v = (ty == G_X) ? v0 + v1 : -sg * (v0 + v1);
+          }
+          if constexpr (ACCL) {
+            tacc[lb * nthr + tid] += v;
+          } else {
+            const double w = wave_sum(v, width);
+            if ((tid & 63) == 0) wacc[(tid >> 6) * (K + 1) + lb] += w;
+          }
+        }
+        __syncthreads();  // the tile buffer is B's from here on
+      }
+    }
+
     // the next tile's loads go out after everything that reads this tile's
     // inputs, so no wait for this tile's data covers them
     if (PF && r + 1 < rounds) fetch(tile_of(r + 1));
@@ -734,7 +808,7 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
     for (int s = 0; s < kMaxStages; ++s) {
       if (s < ns) {
         if (s > 0) {
-          lds_transpose<S, R>(xa, lds, tid, los[s - 1], los[s]);
+          if constexpr (!TWO || POST) lds_transpose<S, R>(xa, lds, tid, los[s - 1], los[s]);
           if constexpr (TWO) lds_transpose<S, R>(xb, lds, tid, los[s - 1], los[s]);
         }
         static_for<R>([&](auto kc) __attribute__((always_inline)) {
@@ -744,7 +818,7 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
           int sl = __builtin_amdgcn_readfirstlane(s * R + kk);
           asm volatile("" : "+s"(sl));
           const int ty = ldc(d->stype, sl), fl = ldc(d->sflag, sl);
-          if constexpr (TWO) {
+          if constexpr (TWO && POST) {
             if (fl & SL_RED) {
               const int lb = ldc(d->slb, sl);
               const double v = red_bit<R, kk>(xb, xa, ty);
@@ -758,7 +832,9 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
           }
           const double ga = ldc(d->sa, sl), gb = ldc(d->sb, sl);
           if constexpr (TWO) {
-            if (fl & SL_ROT_A) gate_bit_s<R, kk>(xa, ty, ga, gb);
+            if constexpr (POST) {
+              if (fl & SL_ROT_A) gate_bit_s<R, kk>(xa, ty, ga, gb);
+            }
             if (fl & SL_ROT_B) gate_bit_s<R, kk>(xb, ty, ga, gb);
           } else {
             gate_bit_s<R, kk>(xa, ty, ga, gb);
@@ -767,7 +843,7 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
       }
     }
     if (ns > 1) {
-      lds_transpose<S, R>(xa, lds, tid, los[ns - 1], t);
+      if constexpr (!TWO || POST) lds_transpose<S, R>(xa, lds, tid, los[ns - 1], t);
       if constexpr (TWO) lds_transpose<S, R>(xb, lds, tid, los[ns - 1], t);
     }
 

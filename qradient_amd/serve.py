@@ -44,7 +44,7 @@ class Request:
         self.future = Future()
 
 
-def passes_per_layer(n, kmax=13, cmin=2):
+def passes_per_layer(n, kmax=13, cmin=3):
     return 1 if n <= kmax else 1 + -(-(n - kmax) // (kmax - cmin))
 
 

@@ -6,7 +6,7 @@ import numpy as np
 from .. import _native
 from ..physical_components import Observable, State
 
-_BACKEND_KEYS = ('dtype', 'entangler', 'device', 'ckpt_every')
+_BACKEND_KEYS = ('dtype', 'entangler', 'device', 'ckpt_every', 'tune')
 
 
 def progbar_range(hide_progbar):
@@ -35,6 +35,7 @@ class ParametrizedCircuit:
             raise ValueError("entangler must be 'cnot' or 'cz'")
         self.device = int(backend.get('device', 0))
         self.ckpt_every = int(backend.get('ckpt_every', 0))
+        self.tune = bool(backend.get('tune', False))
         self.observable = Observable(self.qnum, observable, store_components=use_observable_components)
         self._state = None
         self._gates = None

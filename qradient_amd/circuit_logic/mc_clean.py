@@ -8,7 +8,9 @@ grad[i, q] = dE/d angles[i, q], computed by the adjoint sweep in HIP kernels.
 Backend keywords (not in the reference): ``dtype='complex128'`` (the plain
 method, FP64 throughout) or ``'complex64'`` (complex64 state, FP64 arithmetic
 and reductions); ``entangler='cnot'`` (reference) or ``'cz'``; ``device``;
-``ckpt_every``.
+``ckpt_every``; ``tune=True`` (apply the autotuner's cached tile shape for
+this circuit shape, timing the candidates on the device first if none is
+cached).
 """
 import ctypes
 
@@ -43,6 +45,13 @@ class McClean(ParametrizedCircuit):
                          _native.DTYPES[self.dtype], _native.ENTANGLERS[self.entangler], self.ckpt_every,
                          ctypes.byref(h))
             self._plan = h
+            if self.tune:
+                from .. import autotune
+                shape = autotune.cached('mcclean', self.qnum, self.lnum, self.dtype, self.entangler)
+                if shape is None:
+                    shape, _ = autotune.tune_mcclean(self.qnum, self.lnum, self.dtype, self.entangler,
+                                                     device=self.device)
+                autotune.apply(h, shape)
         return self._plan
 
     def _params(self):

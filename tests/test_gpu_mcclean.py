@@ -132,3 +132,20 @@ def test_checkpoint_uncompute_mode_c64(gpu, n, L, k):
     Eo, go = sv.mcclean_grad_run_adjoint(n, axes, angles, obs)
     assert abs(E - Eo) <= C64_TOL * max(1.0, abs(Eo))
     assert rel_err(g, go) <= C64_TOL
+
+
+def test_tune_applies_cached_shape(gpu, tmp_path, monkeypatch):
+    """tune=True times the tile shapes once, caches the best, and the tuned
+    plan keeps the complex128 parity bar."""
+    from qradient_amd import autotune
+    from qradient_amd.circuit_logic import McClean
+    monkeypatch.setenv('QRADIENT_AMD_TUNE_CACHE', str(tmp_path / 'tune.json'))
+    n, L = 17, 3
+    axes, angles, obs = _case(n, L, 81)
+    c = McClean(n, obs, L, axes=axes, angles=angles, tune=True)
+    E, g = c.grad_run()
+    shape = autotune.cached('mcclean', n, L, 'complex128', 'cnot')
+    assert shape is not None and shape['kmax'] in (11, 12, 13)
+    Eo, go = sv.mcclean_grad_run_adjoint(n, axes, angles, obs)
+    assert abs(E - Eo) <= C128_TOL * max(1.0, abs(Eo))
+    assert rel_err(g, go) <= C128_TOL

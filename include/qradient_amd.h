@@ -124,10 +124,13 @@ int qg_plan_set_profiling(qg_plan* plan, int on);
  * (0 forward pass, 1 reverse two-state pass, 2 forward pass fused with the
  * observable); *count receives the launch count, at most cap are written. */
 int qg_plan_pass_times(const qg_plan* plan, int cap, float* ms, int* kinds, int* count);
-/* Tile shape of a plan: kmax local bits per pass, at least cmin contiguous
- * low bits in high-bit passes, and log2 amplitudes per thread (4 or 5) of the
- * complex64 forward and history-adjoint passes.  Used by the autotuner;
- * rejects shapes that change the number of passes per layer. */
+/* Tile shape of a McClean plan: kmax local bits per pass (4..13), at least
+ * cmin contiguous low bits in high-bit passes, and log2 amplitudes per thread
+ * of the forward and adjoint passes (r_fwd = r_rev = 4).  The plan is laid out
+ * again (its pass count, memory mode and workspace size may change; a
+ * caller-supplied workspace that is too small is an error).  Used by the
+ * autotuner.  Default: the fewest passes per layer that 13 local bits allow,
+ * with the smallest kmax (>= 11 for QG_C64, >= 12 for QG_C128) that keeps it. */
 int qg_plan_tune(qg_plan* plan, int kmax, int cmin, int r_fwd, int r_rev);
 /* on = 1 (default): circuits with n <= kmax run whole in one workgroup each
  * (one launch per call, FP64 throughout); on = 0 uses the pass kernels. */

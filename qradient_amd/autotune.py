@@ -20,6 +20,21 @@ from . import _native
 DEFAULT = {'kmax': 13, 'cmin': 3, 'r_fwd': 4, 'r_rev': 4}
 
 
+def _passes(n, kmax, cmin):
+    return 1 if n <= kmax else 1 + -(-(n - kmax) // (kmax - cmin))
+
+
+def default_shape(n, dtype):
+    """The library's untuned shape: the fewest passes per layer 13 local bits
+    allow, with the smallest tile (>= 11 bits for complex64, 12 for complex128)
+    that keeps that count."""
+    best = _passes(n, 13, 3)
+    for kmax in range(11 if dtype == 'complex64' else 12, 13):
+        if _passes(n, kmax, 3) == best:
+            return dict(DEFAULT, kmax=kmax)
+    return dict(DEFAULT)
+
+
 def cache_path():
     return os.environ.get('QRADIENT_AMD_TUNE_CACHE',
                           os.path.join(os.path.expanduser('~'), '.cache', 'qradient_amd', 'tune.json'))
@@ -55,7 +70,7 @@ def cached(kind, n, L, dtype, entangler):
 
 def candidates(dtype):
     out = []
-    for kmax in (13, 12, 11):
+    for kmax in (13, 12, 11, 10):
         for cmin in (2, 3, 4, 5, 6):
             rs = (4,)
             for r_fwd in rs:
@@ -100,7 +115,7 @@ def tune_mcclean(n, L, dtype='complex64', entangler='cnot', shapes=None, reps=2,
         finally:
             _native.load().qg_plan_destroy(h)
     if not results:
-        return dict(DEFAULT), results
+        return default_shape(n, dtype), results
     best = min(results, key=lambda r: r[1])[0]
     if store:
         c = _load_cache()

@@ -221,6 +221,22 @@ static std::vector<Geo> layer_geometry(int n, int kmax, int cmin) {
   return g;
 }
 
+// Default local bits per pass: the fewest passes per layer that 13 local bits
+// allow, with the smallest tile that keeps that count (at least 11 local bits
+// for complex64, 12 for complex128).  Smaller tiles mean smaller workgroups
+// (2^(K-4) threads): more of them are resident per CU, so one workgroup's
+// barrier waits overlap another's work.
+static int passes_per_layer(int n, int K, int cmin) {
+  return n <= K ? 1 : 1 + (n - K + (K - cmin) - 1) / (K - cmin);
+}
+static int default_kmax(int n, int dtype, int cmin) {
+  const int kmin = dtype == QG_C64 ? 11 : 12;
+  const int best = passes_per_layer(n, kMaxK, cmin);
+  for (int K = kmin; K < kMaxK; ++K)
+    if (passes_per_layer(n, K, cmin) == best) return K;
+  return kMaxK;
+}
+
 // Register bits of a pass by role: complex64 forward and history-adjoint
 // passes hold 32 amplitudes per thread (the adjoint's read-only companion in
 // FP32 registers); passes that propagate two FP64 states hold 16.
@@ -289,7 +305,7 @@ struct qg_plan {
   int kmax = kMaxK, cmin = 3;  // cmin 3: contiguous runs of >= 8 amplitudes in high-bit passes
   int r_fwd = 4, r_rev = 4;  // amplitudes per thread (log2) of forward / history-adjoint passes (complex64)
   int chunk = 1;  // circuits resident at once (QAOA batches)
-  bool fused = true;       // n <= kmax: whole circuit per workgroup (qg_fused.hip)
+  bool fused = true;       // n <= 13: whole circuit per workgroup (qg_fused.hip)
   char* h_arena = nullptr;  // per-call tables, pinned host
   char* d_arena = nullptr;  // and their device copy
   size_t arena_cap = 0;
@@ -564,9 +580,10 @@ static int run_launches(qg_plan* p, int ndesc) {
       // passes without a phase use the kernel variant without phase code
       if (!(d.flags & (F_PHASE_PRE | F_PHASE_POST))) l.phase = 0;
       // adjoint passes whose A is a read-only pass output take the
-      // reduce-first form (POST = false); the rest the general form
-      l.post = l.two && ((d.flags & (F_DRED_POST | F_PHASE_POST | F_OBS_DIAG | F_STORE_A | F_PHASE_PRE)) != 0 ||
-                         !(d.flags & F_LOAD_A) || l.phase != 0);
+      // reduce-first form (POST = false), also with a pre phase that carries
+      // the d/dgamma reduction; the rest the general form
+      l.post = l.two && ((d.flags & (F_DRED_POST | F_PHASE_POST | F_OBS_DIAG | F_STORE_A)) != 0 ||
+                         !(d.flags & F_LOAD_A) || ((d.flags & F_PHASE_PRE) && !(d.flags & F_DRED_PRE)));
       for (int b = 0; b < l.batch; ++b) desc_slots(p->h_desc[l.desc + b], l.R, l.two, l.two && !l.post);
       l.tab = ((d.flags & F_INIT_TAB) ? (int)sizeof(double) * (p->n + 1) : 0) +
               ((l.phase == 1 && (d.flags & (F_PHASE_PRE | F_PHASE_POST))) ? (int)sizeof(double) * 2 * (2 * d.lut_off + 1)
@@ -735,6 +752,7 @@ extern "C" int qg_mcclean_plan(qg_ctx* ctx, int n, int L, int dtype, int entangl
   p->dtype = dtype;
   p->ent = entangler;
   p->ckpt_req = ckpt_every;
+  p->kmax = default_kmax(n, dtype, p->cmin);
   int r = mcclean_layout(p);
   if (r != QG_OK) {
     qg_plan_destroy(p);
@@ -1027,7 +1045,7 @@ static int mcclean_run(qg_plan* p, const int32_t* axes, const double* angles, co
     return fail(QG_ERR_INVALID, "too many observable terms");
   qg_ctx* c = p->ctx;
   HIPCHK(hipSetDevice(c->device));
-  if (p->fused && n <= p->kmax && !ini_state) {
+  if (p->fused && n <= kMaxK && !ini_state) {
     // the whole circuit in one workgroup (qg_fused.hip): gates as (type, cos, sin)
     std::vector<double> res(1 + (size_t)L * n);
     auto fill = [&](int, FusedDescH& d, Arena& ar) -> int {
@@ -1142,6 +1160,7 @@ extern "C" int qg_qaoa_plan(qg_ctx* ctx, int n, int p_layers, int dtype, int bat
   p->L = p_layers;
   p->dtype = dtype;
   p->batch = batch;
+  p->kmax = default_kmax(n, dtype, p->cmin);
   p->geo = layer_geometry(n, p->kmax, p->cmin);
   const int P = (int)p->geo.size();
   p->S = (size_t)(1ull << n) * (dtype == QG_C64 ? 8 : 16);
@@ -1211,7 +1230,7 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
   const int Wi = (int)W;
   qg_ctx* c = p->ctx;
   HIPCHK(hipSetDevice(c->device));
-  if (p->fused && n <= p->kmax) {
+  if (p->fused && n <= kMaxK) {
     const int per = 1 + 2 * L;
     std::vector<double> res((size_t)B * per);
     auto fill = [&](int b, FusedDescH& d, Arena& ar) -> int {
@@ -1324,6 +1343,13 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
       return d;
     };
     const uint32_t phase_flag = phase == 1 ? F_PHASE_LUT : 0u;
+    // With two or more passes per layer the cost phase of layer i + 1 closes
+    // the last pass of layer i (the first layer's opens its first pass), so
+    // the adjoint of that pass takes d/dgamma and undoes the phase in the
+    // natural layout before its reduce-first reductions: psi never has to be
+    // rotated back.  With one pass per layer the phase opens each pass and
+    // the adjoint takes the general form.
+    const bool phase_last = P >= 2;
     // forward
     for (int i = 0; i < L; ++i) {
       for (int g = 0; g < P; ++g) {
@@ -1333,12 +1359,17 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
         for (int j = 0; j < nb; ++j) {
           PassDesc& d = base_desc(cur_geo, j);
           uint32_t f = F_ROT_A | F_STORE_A | phase_flag;
-          if (g == 0) {
+          if (g == 0 && (i == 0 || !phase_last)) {
             f |= (i == 0 ? F_INIT_TAB : F_LOAD_A) | F_PHASE_PRE;
             d.phase_gamma = gammas[(c0 + j) * L + i];
             d.lut = lut_ptr(j, i, 0);
           } else {
             f |= F_LOAD_A;
+          }
+          if (phase_last && g == P - 1 && i + 1 < L) {
+            f |= F_PHASE_POST;
+            d.phase_gamma = gammas[(c0 + j) * L + i + 1];
+            d.lut = lut_ptr(j, i + 1, 0);
           }
           d.a_in = (i == 0 && g == 0) ? nullptr : (g == 0 ? hist(j, i - 1, P - 1) : hist(j, i, g - 1));
           d.a_out = hist(j, i, g);
@@ -1372,11 +1403,25 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
               d.own[lb] = 1;
               out_rows[(size_t)j * per_out + 1 + 2 * i].push_back(row + lb);
             }
-            if (g == 0) {
+            if (!phase_last && g == 0) {
               f |= F_DRED_POST | F_PHASE_POST;
               d.phase_gamma = -gammas[(c0 + j) * L + i];
               d.lut = lut_ptr(j, i, 1);
               out_rows[(size_t)j * per_out + 2 + 2 * i].push_back(row + G.K);
+            }
+            if (phase_last && g == P - 1 && i + 1 < L) {
+              // the forward pass ended with the phase of layer i + 1
+              f |= F_PHASE_PRE | F_DRED_PRE;
+              d.phase_gamma = -gammas[(c0 + j) * L + i + 1];
+              d.lut = lut_ptr(j, i + 1, 1);
+              out_rows[(size_t)j * per_out + 2 + 2 * (i + 1)].push_back(row + G.K);
+            }
+            if (phase_last && g == 0 && i == 0) {
+              // the first layer's phase opened this pass: psi is rotated back
+              // to the initial state times that phase (general form), and
+              // lambda is not needed after it
+              f = (f | F_DRED_POST) & ~F_STORE_B;
+              out_rows[(size_t)j * per_out + 2].push_back(row + G.K);
             }
             d.flags = f;
             for (int k = 0; k <= kMaxK; ++k) cnt[row + k] = gr;

@@ -33,11 +33,12 @@ enum : uint32_t {
   F_SCATTER     = 1u << 9,   // store address = map(x) (ladder scatter, reverse)
   F_CZ_PRE      = 1u << 10,  // multiply by the CZ-ladder sign after load
   F_CZ_POST     = 1u << 11,  // multiply by the CZ-ladder sign before store
-  F_PHASE_PRE   = 1u << 12,  // A *= exp(-i*phase_gamma*D(x)) after load (QAOA cost phase)
-  F_PHASE_POST  = 1u << 13,  // A,B *= exp(-i*phase_gamma*D(x)) before store (inverse phase)
+  F_PHASE_PRE   = 1u << 12,  // A (and B) *= exp(-i*phase_gamma*D(x)) after load (QAOA cost phase)
+  F_PHASE_POST  = 1u << 13,  // A (and B) *= exp(-i*phase_gamma*D(x)) before store
   F_DRED_POST   = 1u << 14,  // extra += 2*sum D(x) Im(conj(B)A) before the post phase (dE/dgamma)
   F_OBS_DIAG    = 1u << 15,  // B = D(x)*A at the end; extra += D(x)|A|^2 (lambda = O psi, E)
   F_PHASE_LUT   = 1u << 16,  // D(x) is integer valued: phase from lut[D + lut_off]
+  F_DRED_PRE    = 1u << 17,  // (adjoint, with F_PHASE_PRE) extra += 2*sum D(x) Im(conj(B)A) before the pre phase
 };
 
 // One term of the diagonal quadratic form D(x) = sum w * z_a(x) [* z_b(x)],

@@ -4,7 +4,8 @@
 
 namespace qg {
 
-// post: the pass has diagonal ops after its gates (adjoint passes only).
+// post: an adjoint pass of the general form (both states rotated); forward
+// passes take post = false (their phase code runs before or after the gates).
 template <int R>
 static hipError_t launch_c64_r(bool two, int phase, bool post, const PassDesc* d, int K, int grid, int batch,
                                int threads, bool big, hipStream_t st, int* occ, int tab) {
@@ -18,8 +19,12 @@ static hipError_t launch_c64_r(bool two, int phase, bool post, const PassDesc* d
     if (post) return launch_pass_t<float, R, true, 0, true>(d, K, grid, batch, threads, big, st, occ, tab);
     return launch_pass_t<float, R, true, 0, false>(d, K, grid, batch, threads, big, st, occ, tab);
   }
-  if (phase == 1) return launch_pass_t<float, R, true, 1, true>(d, K, grid, batch, threads, big, st, occ, tab);
-  return launch_pass_t<float, R, true, 2, true>(d, K, grid, batch, threads, big, st, occ, tab);
+  if (phase == 1) {
+    if (post) return launch_pass_t<float, R, true, 1, true>(d, K, grid, batch, threads, big, st, occ, tab);
+    return launch_pass_t<float, R, true, 1, false>(d, K, grid, batch, threads, big, st, occ, tab);
+  }
+  if (post) return launch_pass_t<float, R, true, 2, true>(d, K, grid, batch, threads, big, st, occ, tab);
+  return launch_pass_t<float, R, true, 2, false>(d, K, grid, batch, threads, big, st, occ, tab);
 }
 
 hipError_t launch_pass_c64(int R, bool two, int phase, bool post, const PassDesc* d, int K, int grid, int batch,
@@ -39,7 +44,7 @@ hipError_t set_attributes_c64() {
   if (e == hipSuccess) e = set_attr_t<float, R, TWO, PH, POST>();
 #define QG_R(R)                                                                               \
   QG_A(R, false, 0, false) QG_A(R, false, 1, false) QG_A(R, false, 2, false) QG_A(R, true, 0, false) \
-  QG_A(R, true, 0, true) QG_A(R, true, 1, true) QG_A(R, true, 2, true)
+  QG_A(R, true, 0, true) QG_A(R, true, 1, true) QG_A(R, true, 2, true) QG_A(R, true, 1, false) QG_A(R, true, 2, false)
   QG_R(4) QG_R(3) QG_R(2) QG_R(1)
 #undef QG_R
 #undef QG_A

@@ -570,6 +570,11 @@ __device__ __forceinline__ void tile_addresses(const PassDesc* __restrict__ d, b
 //        reduction Im<B|P_q|A> is taken in the natural layout before any gate
 //        is undone (each P_q commutes with every gate of the pass, so the pass
 //        output is a valid frame for all of them), and only B is rotated.
+//        With a phase (PH != 0) the reduce-first form is the adjoint of a
+//        forward pass that ended with the phase: d/dgamma is taken at the pass
+//        output, both states are taken back through the phase in the natural
+//        layout, and the pass continues as above.
+//   Forward passes (TWO false) with a phase apply it before or after the gates.
 //
 // The next tile's loads are issued before the current tile is processed (the
 // raw storage-type values wait in registers), except in complex128 adjoint
@@ -584,9 +589,17 @@ template <class S, int R, bool TWO, int PH, bool POST, bool BIG>
 __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* lds, uint32_t tid_in, uint32_t blk,
                                          uint32_t nblk) {
   constexpr int A = 1 << R;
-  constexpr bool PF = !(TWO && (sizeof(S) == 8 || POST));
+  constexpr bool PF = !(TWO && (sizeof(S) == 8 || POST || PH != 0));
   constexpr bool ACCL = TWO && sizeof(S) == 4;
-  typedef double TAr;
+  // diagonal ops after the gates: adjoint passes of the general form, and
+  // forward passes with a phase (which may sit before or after the gates)
+  constexpr bool POST_DIAG = POST || (!TWO && PH != 0);
+  // the workgroup's FP64 sum of the diagonal reductions (d/dgamma, E)
+  constexpr bool DSUM = TWO && (POST || PH != 0);
+  // Register type of A.  A reduce-first complex64 pass only reads A (the
+  // FP32 pass output) in its FP64 reductions, so it stays in FP32 registers
+  // (exact, or rounded once after a pre phase); everywhere else A is FP64.
+  typedef typename std::conditional<TWO && !POST && sizeof(S) == 4, float, double>::type TAr;
   const int n = d->n, K = d->K, t = d->t, c = d->c, p = d->p;
   const uint32_t flags = d->flags;
   const uint32_t ntiles = 1u << (n - K);
@@ -624,7 +637,7 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
   double* wacc = reinterpret_cast<double*>(reinterpret_cast<char*>(lds) + (sizeof(S) * 2u << K));
   double* tacc = wacc + nwaves * (kMaxK + 1);
   double* tab = tacc + (ACCL ? K * (int)nthr : 0);
-  const bool reduce = TWO && (flags & (F_REDUCE | F_DRED_POST | F_OBS_DIAG)) && d->row_base >= 0;
+  const bool reduce = TWO && (flags & (F_REDUCE | F_DRED_POST | F_DRED_PRE | F_OBS_DIAG)) && d->row_base >= 0;
   const bool reduce_bits = reduce && (flags & F_REDUCE);
   const uint32_t wave = tid_in >> 6;
   const bool lane0 = (tid_in & 63u) == 0u;
@@ -646,7 +659,10 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
   }
   double dacc = 0.0;
   const bool gather = (flags & F_GATHER) != 0;
-  const bool load_a = !(flags & F_INIT_TAB);
+  // reduce-first adjoint: A is a loaded pass output, only read by the
+  // reductions (never generated, rotated, sign-flipped after them or stored)
+  constexpr bool RF = TWO && !POST;
+  const bool load_a = RF || !(flags & F_INIT_TAB);
   const bool load_b = TWO && (flags & F_LOAD_B);
   // A's gates only matter for later reductions or for A's own use after the gates
   const bool a_used_after = !TWO || (flags & (F_STORE_A | F_DRED_POST | F_OBS_DIAG));
@@ -669,16 +685,24 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
     asm volatile("" : "+v"(tid));
     uint32_t adr[A];
     tile_addresses<R>(d, gather, gl_tile(tt, c, p, K) | gl_local(tid, c, p), rpos, rbit, n, adr);
+    // every path defines every register (zeros when a state is not loaded),
+    // so no value is carried from one tile to the next
     if (load_a) {
 #pragma unroll
       for (int a = 0; a < A; ++a)
         ra[a] = BIG ? Stor<S>::template ld_big<S>(d->a_in, adr[a]) : Stor<S>::template ld<S>(d->a_in, adr[a]);
+    } else {
+#pragma unroll
+      for (int a = 0; a < A; ++a) ra[a] = {(S)0, (S)0};
     }
     if constexpr (TWO) {
       if (load_b) {
 #pragma unroll
         for (int a = 0; a < A; ++a)
           rb[a] = BIG ? Stor<S>::template ld_big<S>(d->b_in, adr[a]) : Stor<S>::template ld<S>(d->b_in, adr[a]);
+      } else {
+#pragma unroll
+        for (int a = 0; a < A; ++a) rb[a] = {(S)0, (S)0};
       }
     }
   };
@@ -704,7 +728,7 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
     }
     if constexpr (TWO) {
 #pragma unroll
-      for (int a = 0; a < A; ++a) xb[a] = load_b ? Cd{(double)rb[a].re, (double)rb[a].im} : Cd{0.0, 0.0};
+      for (int a = 0; a < A; ++a) xb[a] = Cd{(double)rb[a].re, (double)rb[a].im};
     }
 
     // ---- diagonal ops in the natural layout, before the gates --------------
@@ -720,14 +744,31 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
       }
     }
     if constexpr (PH != 0) {
-      if (flags & F_PHASE_PRE) {
+      // a reduce-first pass runs with a phase only when the phase opens it
+      // (the branch is compiled out: a join here costs register copies)
+      if (RF || (flags & F_PHASE_PRE)) {
+        // adjoint of a pass whose forward ends with the phase: d/dgamma at the
+        // pass output, then both states are taken back through the phase
         DiagParts<R> dp;
         diag_prepare<R>(d, gthr, rpos, dp);
+        double xacc = 0.0;
 #pragma unroll
         for (int a = 0; a < A; ++a) {
-          xa[a] = cmul(phase_of<PH>(d, tab, diag_value<R>(dp, a)), xa[a]);
+          const double D = diag_value<R>(dp, a);
+          if constexpr (TWO) {
+            if (RF || (flags & F_DRED_PRE)) {
+              xacc += 2.0 * D * imcj(xb[a], xa[a]);
+              // taken here, from the unphased states (otherwise the sum may
+              // be sunk past the loop, keeping both copies of the states live)
+              asm volatile("" : "+v"(xacc));
+            }
+          }
+          const Cd ph = phase_of<PH>(d, tab, D);
+          xa[a] = cmul(ph, xa[a]);
+          if constexpr (TWO) xb[a] = cmul(ph, xb[a]);
           __builtin_amdgcn_sched_barrier(0);
         }
+        dacc += xacc;
       }
     }
     // ---- reduce-first adjoint: all reductions in the natural layout -------
@@ -847,10 +888,16 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
       if constexpr (TWO) lds_transpose<S, R>(xb, lds, tid, los[ns - 1], t);
     }
 
+    // index-derived values after the stages come from a laundered base, so
+    // the compiler does not keep the per-amplitude indices of the pre-gate
+    // ops live across the stages
+    uint32_t gst = gthr;
+    asm volatile("" : "+v"(gst));
+
     // ---- diagonal ops in the natural layout, after the gates --------------
-    if (POST && (flags & (F_DRED_POST | F_PHASE_POST | F_OBS_DIAG))) {
+    if (POST_DIAG && (flags & (F_DRED_POST | F_PHASE_POST | F_OBS_DIAG))) {
       DiagParts<R> dp;
-      diag_prepare<R>(d, gthr, rpos, dp);
+      diag_prepare<R>(d, gst, rpos, dp);
       double xacc = 0.0;
 #pragma unroll
       for (int a = 0; a < A; ++a) {
@@ -873,7 +920,9 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
           }
         }
         // one amplitude at a time: keeps the scheduler from evaluating all
-        // sixteen diagonal values and phases ahead (register pressure)
+        // sixteen diagonal values and phases ahead, and the sum from being
+        // sunk past the loop (register pressure)
+        asm volatile("" : "+v"(xacc));
         __builtin_amdgcn_sched_barrier(0);
       }
       dacc += xacc;
@@ -881,22 +930,20 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
     if (flags & F_CZ_POST) {
 #pragma unroll
       for (int a = 0; a < A; ++a) {
-        uint32_t x = gthr;
+        uint32_t x = gst;
 #pragma unroll
         for (int k = 0; k < R; ++k)
           if (a & (1 << k)) x |= rbit[k];
         const double sg = cz_sign(x, n);
-        xa[a] = {(TAr)sg * xa[a].re, (TAr)sg * xa[a].im};
+        if constexpr (!RF) xa[a] = {(TAr)sg * xa[a].re, (TAr)sg * xa[a].im};
         if constexpr (TWO) xb[a] = {sg * xb[a].re, sg * xb[a].im};
       }
     }
 
-    // ---- store: addresses recomputed from a laundered base, not kept live ----
-    uint32_t gst = gthr;
-    asm volatile("" : "+v"(gst));
+    // ---- store: addresses recomputed from the laundered base ----------------
     uint32_t adr[A];
     tile_addresses<R>(d, (flags & F_SCATTER) != 0, gst, rpos, rbit, n, adr);
-    if (flags & F_STORE_A) {
+    if (!RF && (flags & F_STORE_A)) {
 #pragma unroll
       for (int a = 0; a < A; ++a) {
         if (BIG) Stor<S>::st_big(d->a_out, adr[a], xa[a]);
@@ -924,7 +971,7 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
       }
     }
   }
-  if (POST && (flags & (F_DRED_POST | F_OBS_DIAG))) {
+  if (DSUM && (flags & (F_DRED_POST | F_DRED_PRE | F_OBS_DIAG))) {
     const double v = wave_sum(dacc, width);
     if (lane0) wacc[wave * (K + 1) + K] = v;
   }
@@ -932,7 +979,7 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
   for (uint32_t i = tid_in; i < (uint32_t)(K + 1); i += nthr) {
     int row = -1;
     if (i == (uint32_t)K) {
-      if (flags & (F_DRED_POST | F_OBS_DIAG)) row = d->row_base + K;
+      if (flags & (F_DRED_POST | F_DRED_PRE | F_OBS_DIAG)) row = d->row_base + K;
     } else if ((flags & F_REDUCE) && d->own[i]) {
       row = d->row_base + (int)i;
     }

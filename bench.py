@@ -203,7 +203,9 @@ def bench_large(args, sync):
         dt = time.perf_counter() - t0
         k = ctypes.c_int32()
         _native.call('qg_plan_ckpt_every', circ._plan, ctypes.byref(k))
-        res[dtype] = dict(ms=dt * 1e3, E=E, g=g, k=int(k.value))
+        t64 = ctypes.c_int32()
+        _native.call('qg_plan_lds64', circ._plan, ctypes.byref(t64))
+        res[dtype] = dict(ms=dt * 1e3, E=E, g=g, k=int(k.value), lds64=int(t64.value))
         circ._destroy_plan()
     c64, c128 = res['complex64'], res['complex128']
     rel = float(np.linalg.norm(c64['g'] - c128['g']) / np.linalg.norm(c128['g']))
@@ -213,7 +215,7 @@ def bench_large(args, sync):
                  'checkpoints every %d layers)' % c64['k'], 1e3 / c64['ms'], 'circuits/s', 1, 0, c64['ms'],
                  'c64 (FP64 arithmetic and reductions)',
                  {'workload': 'mcclean_grad_run_n30_L100_zzchain', 'qubits': n, 'layers': L, 'seed': SEED,
-                  'ckpt_every': c64['k']},
+                  'ckpt_every': c64['k'], 'fp64_lds_tiles': bool(c64['lds64'])},
                  c128_plain={'ms_per_step': c128['ms'], 'ckpt_every': c128['k']},
                  c64_vs_c128_grad_rel_err=rel, effective_gbs=b_alg / (c64['ms'] * 1e-3) / 1e9, b_alg_bytes=b_alg)
 

@@ -135,6 +135,13 @@ int qg_plan_tune(qg_plan* plan, int kmax, int cmin, int r_fwd, int r_rev);
 /* on = 1 (default): circuits with n <= kmax run whole in one workgroup each
  * (one launch per call, FP64 throughout); on = 0 uses the pass kernels. */
 int qg_plan_set_fused(qg_plan* plan, int on);
+/* LDS tile precision of a QG_C64 McClean plan: mode 1 keeps the pass tiles
+ * in FP64 (the state is rounded to complex64 once per pass, when stored),
+ * 0 in FP32 (also rounded at each of a pass's LDS transposes; faster), -1
+ * (default) picks FP64 when the circuit has more than 200 passes.  The plan
+ * is laid out again.  No counterpart in the reference (complex128 only). */
+int qg_plan_set_lds64(qg_plan* plan, int mode);
+int qg_plan_lds64(const qg_plan* plan, int* out);
 /* The checkpoint period a McClean plan chose (0: full per-pass history). */
 int qg_plan_ckpt_every(qg_plan* plan, int* out);
 int qg_plan_destroy(qg_plan* plan);

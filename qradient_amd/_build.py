@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 BUILD = os.path.join(HERE, '_build')
 LIB = os.path.join(HERE, 'libqradient_amd.so')
-SOURCES = ['qg_kernels_c64.hip', 'qg_kernels_c128.hip', 'qg_finalize.hip', 'qg_fused.hip', 'qg_simple.hip',
+SOURCES = ['qg_kernels_c64.hip', 'qg_kernels_c64x.hip', 'qg_kernels_c128.hip', 'qg_finalize.hip', 'qg_fused.hip', 'qg_simple.hip',
            'qg_api.cpp']
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC',

@@ -304,6 +304,8 @@ struct qg_plan {
   size_t off_work = 0, off_ckpt = 0;
   int kmax = kMaxK, cmin = 3;  // cmin 3: contiguous runs of >= 8 amplitudes in high-bit passes
   int r_fwd = 4, r_rev = 4;  // amplitudes per thread (log2) of forward / history-adjoint passes (complex64)
+  int lds64_req = -1;  // complex64 McClean: FP64 LDS tiles (-1 auto, 0 no, 1 yes)
+  bool lds64 = false;
   int chunk = 1;  // circuits resident at once (QAOA batches)
   bool fused = true;       // n <= 13: whole circuit per workgroup (qg_fused.hip)
   char* h_arena = nullptr;  // per-call tables, pinned host
@@ -365,6 +367,10 @@ static int ensure_workspace(qg_plan* p) {
   return QG_OK;
 }
 
+// Kernel storage code of a plan's passes: complex128 (0), complex64 (1), or
+// complex64 with FP64 LDS tiles (2).
+static int kdtype(const qg_plan* p) { return (p->dtype == QG_C64 && p->lds64) ? 2 : p->dtype; }
+
 static size_t lds_bytes_of(int dtype, int K, int threads) {  // the larger (adjoint) variant
   return (size_t)(1u << K) * (dtype == QG_C64 ? 8 : 16) + (size_t)((threads + 63) / 64) * (kMaxK + 1) * 8 +
          (dtype == QG_C64 ? (size_t)K * threads * 8 : 0);
@@ -375,10 +381,10 @@ static size_t lds_bytes_of(int dtype, int K, int threads) {  // the larger (adjo
 // a power of two and at most one per tile.
 static int grid_for_pass(const qg_plan* p, const Geo& g, bool two, bool post, int phase) {
   int occ = 0;
-  if (launch_pass(p->dtype, g.R, two, phase, post, nullptr, g.K, 0, 1, g.threads, p->n, nullptr, &occ, 0) !=
+  if (launch_pass(kdtype(p), g.R, two, phase, post, nullptr, g.K, 0, 1, g.threads, p->n, nullptr, &occ, 0) !=
           hipSuccess ||
       occ < 1) {
-    const size_t lds = lds_bytes_of(p->dtype, g.K, g.threads);
+    const size_t lds = lds_bytes_of(kdtype(p), g.K, g.threads);
     occ = (int)std::max<size_t>(1, (160 * 1024) / lds);
     occ = std::min(occ, std::max(1, 512 / g.threads));
   }
@@ -603,7 +609,7 @@ static int run_launches(qg_plan* p, int ndesc) {
     const Launch& l = p->launches[k];
     if (p->profile) HIPCHK(hipEventRecord(p->pev[k], c->stream));
     if (l.kind == 0) {
-      HIPCHK(launch_pass(p->dtype, l.R, l.two, l.phase, l.post, p->d_desc + l.desc, l.K, l.grid, l.batch,
+      HIPCHK(launch_pass(kdtype(p), l.R, l.two, l.phase, l.post, p->d_desc + l.desc, l.K, l.grid, l.batch,
                          l.threads, p->n, c->stream, nullptr, l.tab));
     }
   }
@@ -685,6 +691,11 @@ static int mcclean_layout(qg_plan* p) {
   p->geo = layer_geometry(n, p->kmax, p->cmin);
   const int P = (int)p->geo.size();
   p->S = (size_t)(1ull << n) * (p->dtype == QG_C64 ? 8 : 16);
+  // complex64 states are rounded at every LDS transpose of a pass with FP32
+  // tiles; beyond about 200 passes these roundings would take the gradient
+  // past a 1e-6 relative error, so long circuits use FP64 tiles (rounding
+  // once per pass, at the store)
+  p->lds64 = p->dtype == QG_C64 && n >= 4 && (p->lds64_req == 1 || (p->lds64_req < 0 && P * L > 200));
   size_t free_b = 0, total_b = 0;
   hipMemGetInfo(&free_b, &total_b);
   if (p->ws_owned && p->ws) free_b += p->ws_bytes;
@@ -1711,29 +1722,48 @@ extern "C" int qg_state_scale(qg_state* s, double re, double im) {
   return QG_OK;
 }
 
-extern "C" int qg_plan_tune(qg_plan* p, int kmax, int cmin, int r_fwd, int r_rev) {
-  if (!p) return fail(QG_ERR_INVALID, "NULL plan");
-  if (kmax < 4 || kmax > kMaxK) return fail(QG_ERR_INVALID, "kmax must be in [4, 13]");
-  if (cmin < 0 || cmin >= kmax) return fail(QG_ERR_INVALID, "cmin must be in [0, kmax)");
-  if (r_fwd != 4 || r_rev != 4) return fail(QG_ERR_INVALID, "r_fwd, r_rev must be 4 (16 amplitudes per thread)");
-  if (p->kind != PK_MCCLEAN) return fail(QG_ERR_UNSUPPORTED, "tuning applies to McClean plans");
+// Lays a McClean plan out again after a change of its shape parameters.
+static int relayout(qg_plan* p) {
   HIPCHK(hipSetDevice(p->ctx->device));
   HIPCHK(hipStreamSynchronize(p->ctx->stream));
   const size_t old_ws = p->ws_bytes;
-  p->kmax = kmax;
-  p->cmin = cmin;
-  p->r_fwd = r_fwd;
-  p->r_rev = r_rev;
   free_small(p);
   CHK(mcclean_layout(p));
   if (p->ws && p->ws_bytes > old_ws) {
-    if (!p->ws_owned) return fail(QG_ERR_INVALID, "tuned layout needs a larger workspace than the one supplied");
+    if (!p->ws_owned) return fail(QG_ERR_INVALID, "the new layout needs a larger workspace than the one supplied");
     hipFree(p->ws);
     p->ws = nullptr;
     p->ws_owned = false;
   }
   p->last_state_ptr = nullptr;
   return QG_OK;
+}
+
+extern "C" int qg_plan_set_lds64(qg_plan* p, int mode) {
+  if (!p) return fail(QG_ERR_INVALID, "NULL plan");
+  if (mode < -1 || mode > 1) return fail(QG_ERR_INVALID, "mode must be -1, 0 or 1");
+  if (p->kind != PK_MCCLEAN) return fail(QG_ERR_UNSUPPORTED, "FP64 tiles apply to McClean plans");
+  p->lds64_req = mode;
+  return relayout(p);
+}
+
+extern "C" int qg_plan_lds64(const qg_plan* p, int* out) {
+  if (!p || !out) return fail(QG_ERR_INVALID, "NULL argument");
+  *out = p->lds64 ? 1 : 0;
+  return QG_OK;
+}
+
+extern "C" int qg_plan_tune(qg_plan* p, int kmax, int cmin, int r_fwd, int r_rev) {
+  if (!p) return fail(QG_ERR_INVALID, "NULL plan");
+  if (kmax < 4 || kmax > kMaxK) return fail(QG_ERR_INVALID, "kmax must be in [4, 13]");
+  if (cmin < 0 || cmin >= kmax) return fail(QG_ERR_INVALID, "cmin must be in [0, kmax)");
+  if (r_fwd != 4 || r_rev != 4) return fail(QG_ERR_INVALID, "r_fwd, r_rev must be 4 (16 amplitudes per thread)");
+  if (p->kind != PK_MCCLEAN) return fail(QG_ERR_UNSUPPORTED, "tuning applies to McClean plans");
+  p->kmax = kmax;
+  p->cmin = cmin;
+  p->r_fwd = r_fwd;
+  p->r_rev = r_rev;
+  return relayout(p);
 }
 
 extern "C" int qg_plan_set_fused(qg_plan* p, int on) {

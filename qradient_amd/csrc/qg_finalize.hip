@@ -10,8 +10,11 @@ hipError_t launch_pass_c64(int R, bool two, int phase, bool post, const PassDesc
                            int threads, bool big, hipStream_t st, int* occ, int tab);
 hipError_t launch_pass_c128(int R, bool two, int phase, bool post, const PassDesc* d, int K, int grid, int batch,
                             int threads, bool big, hipStream_t st, int* occ, int tab);
+hipError_t launch_pass_c64x(int R, bool two, int phase, bool post, const PassDesc* d, int K, int grid, int batch,
+                            int threads, bool big, hipStream_t st, int* occ, int tab);
 hipError_t set_attributes_c64();
 hipError_t set_attributes_c128();
+hipError_t set_attributes_c64x();
 
 // out[i] = scale[i] * sum over the rows r of output i of sum_{w < cnt[r]} part[r*stride + w].
 // One workgroup per output; each thread sums a fixed strided subset, then a
@@ -46,18 +49,21 @@ hipError_t launch_finalize(int nout, const int* ptr, const int* rows, const int*
 hipError_t set_pass_attributes() {
   hipError_t e = set_attributes_c64();
   if (e == hipSuccess) e = set_attributes_c128();
+  if (e == hipSuccess) e = set_attributes_c64x();
   return e;
 }
 
-// dtype 1: complex64 storage, 0: complex128.  post: the pass has diagonal
-// ops after its gates.  With occ != NULL the pass is not launched: *occ
+// dtype 1: complex64 storage, 0: complex128, 2: complex64 storage with FP64
+// LDS tiles (phase-free passes only).  post: an adjoint pass of the general
+// form.  With occ != NULL the pass is not launched: *occ
 // receives the workgroups per CU.  tab: bytes of the table the pass stages in
 // LDS.
 hipError_t launch_pass(int dtype, int R, bool two, int phase, bool post, const PassDesc* d, int K, int grid,
                        int batch, int threads, int n, hipStream_t st, int* occ, int tab) {
   // byte offsets of a 2^n state: 32-bit addressing below 4 GiB
-  const bool big = (n + (dtype == 1 ? 3 : 4)) > 32;
+  const bool big = (n + (dtype == 0 ? 4 : 3)) > 32;
   if (dtype == 1) return launch_pass_c64(R, two, phase, post, d, K, grid, batch, threads, big, st, occ, tab);
+  if (dtype == 2) return launch_pass_c64x(R, two, phase, post, d, K, grid, batch, threads, big, st, occ, tab);
   return launch_pass_c128(R, two, phase, post, d, K, grid, batch, threads, big, st, occ, tab);
 }
 

@@ -163,6 +163,23 @@ struct Stor<double> {
   }
 };
 
+// Storage of a pass kernel: the element type of the states in HBM (G) and
+// of the LDS tiles (L).  float / double: complex64 / complex128 throughout;
+// F32L64: complex64 states with FP64 LDS tiles, so a complex64 state is
+// rounded only when it is stored (once per pass) instead of at every
+// transpose as well.
+struct F32L64 {};
+template <class S>
+struct Store {
+  typedef S G;
+  typedef S L;
+};
+template <>
+struct Store<F32L64> {
+  typedef float G;
+  typedef double L;
+};
+
 // Uniform-index read of a read-only table through the constant address space:
 // compiles to a scalar load (lgkmcnt only), so it never waits on the vector
 // memory counter that tracks the tile's global loads and stores.
@@ -558,7 +575,8 @@ __device__ __forceinline__ void tile_addresses(const PassDesc* __restrict__ d, b
 
 // ---------------------------------------------------------------------------
 // One pass over the tiles assigned to this workgroup.
-//   S  : storage and LDS element type (float: complex64, double: complex128)
+//   S  : storage (float: complex64, double: complex128, F32L64: complex64
+//        states with FP64 LDS tiles; see Store)
 //   R  : register bits per thread (2^R amplitudes)
 //   TWO: an adjoint pass with the companion state A and the propagated B
 //   PH : diagonal phase method (0 none, 1 table, 2 sincos)
@@ -589,8 +607,10 @@ template <class S, int R, bool TWO, int PH, bool POST, bool BIG>
 __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* lds, uint32_t tid_in, uint32_t blk,
                                          uint32_t nblk) {
   constexpr int A = 1 << R;
-  constexpr bool PF = !(TWO && (sizeof(S) == 8 || POST || PH != 0));
-  constexpr bool ACCL = TWO && sizeof(S) == 4;
+  typedef typename Store<S>::G G;  // HBM element type
+  typedef typename Store<S>::L LT; // LDS tile element type
+  constexpr bool PF = !(TWO && (sizeof(G) == 8 || POST || PH != 0));
+  constexpr bool ACCL = TWO && sizeof(LT) == 4;
   // diagonal ops after the gates: adjoint passes of the general form, and
   // forward passes with a phase (which may sit before or after the gates)
   constexpr bool POST_DIAG = POST || (!TWO && PH != 0);
@@ -599,7 +619,7 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
   // Register type of A.  A reduce-first complex64 pass only reads A (the
   // FP32 pass output) in its FP64 reductions, so it stays in FP32 registers
   // (exact, or rounded once after a pre phase); everywhere else A is FP64.
-  typedef typename std::conditional<TWO && !POST && sizeof(S) == 4, float, double>::type TAr;
+  typedef typename std::conditional<TWO && !POST && sizeof(G) == 4, float, double>::type TAr;
   const int n = d->n, K = d->K, t = d->t, c = d->c, p = d->p;
   const uint32_t flags = d->flags;
   const uint32_t ntiles = 1u << (n - K);
@@ -634,7 +654,7 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
   // each wave is the only writer of its row), for ACCL the per-thread
   // accumulators [local bit][thread], then the staged table (the initial
   // state's popcount table or the phase table).
-  double* wacc = reinterpret_cast<double*>(reinterpret_cast<char*>(lds) + (sizeof(S) * 2u << K));
+  double* wacc = reinterpret_cast<double*>(reinterpret_cast<char*>(lds) + (sizeof(LT) * 2u << K));
   double* tacc = wacc + nwaves * (kMaxK + 1);
   double* tab = tacc + (ACCL ? K * (int)nthr : 0);
   const bool reduce = TWO && (flags & (F_REDUCE | F_DRED_POST | F_DRED_PRE | F_OBS_DIAG)) && d->row_base >= 0;
@@ -676,8 +696,8 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
     return xcd ? (blk % 8) * per_x + r * slots + blk / 8 : blk + r * nblk;
   };
 
-  Cx<S> ra[A];
-  Cx<S> rb[TWO ? A : 1];
+  Cx<G> ra[A];
+  Cx<G> rb[TWO ? A : 1];
   // Issues the loads of tile tt into ra / rb (addresses derived from a
   // laundered thread index so they are not kept live across the tile).
   auto fetch = [&](uint32_t tt) __attribute__((always_inline)) {
@@ -690,19 +710,19 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
     if (load_a) {
 #pragma unroll
       for (int a = 0; a < A; ++a)
-        ra[a] = BIG ? Stor<S>::template ld_big<S>(d->a_in, adr[a]) : Stor<S>::template ld<S>(d->a_in, adr[a]);
+        ra[a] = BIG ? Stor<G>::template ld_big<G>(d->a_in, adr[a]) : Stor<G>::template ld<G>(d->a_in, adr[a]);
     } else {
 #pragma unroll
-      for (int a = 0; a < A; ++a) ra[a] = {(S)0, (S)0};
+      for (int a = 0; a < A; ++a) ra[a] = {(G)0, (G)0};
     }
     if constexpr (TWO) {
       if (load_b) {
 #pragma unroll
         for (int a = 0; a < A; ++a)
-          rb[a] = BIG ? Stor<S>::template ld_big<S>(d->b_in, adr[a]) : Stor<S>::template ld<S>(d->b_in, adr[a]);
+          rb[a] = BIG ? Stor<G>::template ld_big<G>(d->b_in, adr[a]) : Stor<G>::template ld<G>(d->b_in, adr[a]);
       } else {
 #pragma unroll
-        for (int a = 0; a < A; ++a) rb[a] = {(S)0, (S)0};
+        for (int a = 0; a < A; ++a) rb[a] = {(G)0, (G)0};
       }
     }
   };
@@ -799,10 +819,10 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
           m = fmd(-xb[a].im, xa[a].re, m);
         }
         const double msum = m0 + m1;
-        constexpr int SH = sizeof(S) == 4 ? 3 : 4;
+        constexpr int SH = sizeof(LT) == 4 ? 3 : 4;
 #pragma unroll
         for (int a = 0; a < A; ++a)
-          Stor<S>::lds_st_b(lds, swz<S>(tid | ((uint32_t)a << t)) << SH, xa[a]);
+          Stor<LT>::lds_st_b(lds, swz<LT>(tid | ((uint32_t)a << t)) << SH, xa[a]);
         __syncthreads();
 #pragma unroll 1
         for (int lb = 0; lb < t; ++lb) {
@@ -813,11 +833,11 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
           if (ty == G_Z) {
             v = sg * msum;
           } else {
-            const uint32_t pb = swz<S>(tid ^ (1u << lb)) << SH;
+            const uint32_t pb = swz<LT>(tid ^ (1u << lb)) << SH;
             double v0 = 0.0, v1 = 0.0;
 #pragma unroll
             for (int a = 0; a < A; ++a) {
-              const Cd pa = Stor<S>::template lds_ld_b<double>(lds, pb ^ (swz<S>((uint32_t)a << t) << SH));
+              const Cd pa = Stor<LT>::template lds_ld_b<double>(lds, pb ^ (swz<LT>((uint32_t)a << t) << SH));
               double& acc = (a & 1) ? v1 : v0;
               if (ty == G_X) {  // Im(conj(B_x) A_{x^q})
                 acc = fmd(xb[a].re, pa.im, acc);
@@ -849,8 +869,8 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
     for (int s = 0; s < kMaxStages; ++s) {
       if (s < ns) {
         if (s > 0) {
-          if constexpr (!TWO || POST) lds_transpose<S, R>(xa, lds, tid, los[s - 1], los[s]);
-          if constexpr (TWO) lds_transpose<S, R>(xb, lds, tid, los[s - 1], los[s]);
+          if constexpr (!TWO || POST) lds_transpose<LT, R>(xa, lds, tid, los[s - 1], los[s]);
+          if constexpr (TWO) lds_transpose<LT, R>(xb, lds, tid, los[s - 1], los[s]);
         }
         static_for<R>([&](auto kc) __attribute__((always_inline)) {
           constexpr int kk = decltype(kc)::value;
@@ -884,8 +904,8 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
       }
     }
     if (ns > 1) {
-      if constexpr (!TWO || POST) lds_transpose<S, R>(xa, lds, tid, los[ns - 1], t);
-      if constexpr (TWO) lds_transpose<S, R>(xb, lds, tid, los[ns - 1], t);
+      if constexpr (!TWO || POST) lds_transpose<LT, R>(xa, lds, tid, los[ns - 1], t);
+      if constexpr (TWO) lds_transpose<LT, R>(xb, lds, tid, los[ns - 1], t);
     }
 
     // index-derived values after the stages come from a laundered base, so
@@ -946,16 +966,16 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
     if (!RF && (flags & F_STORE_A)) {
 #pragma unroll
       for (int a = 0; a < A; ++a) {
-        if (BIG) Stor<S>::st_big(d->a_out, adr[a], xa[a]);
-        else Stor<S>::st(d->a_out, adr[a], xa[a]);
+        if (BIG) Stor<G>::st_big(d->a_out, adr[a], xa[a]);
+        else Stor<G>::st(d->a_out, adr[a], xa[a]);
       }
     }
     if constexpr (TWO) {
       if (flags & F_STORE_B) {
 #pragma unroll
         for (int a = 0; a < A; ++a) {
-          if (BIG) Stor<S>::st_big(d->b_out, adr[a], xb[a]);
-          else Stor<S>::st(d->b_out, adr[a], xb[a]);
+          if (BIG) Stor<G>::st_big(d->b_out, adr[a], xb[a]);
+          else Stor<G>::st(d->b_out, adr[a], xb[a]);
         }
       }
     }
@@ -1002,9 +1022,14 @@ __global__ __launch_bounds__(512) void k_pass(const PassDesc* __restrict__ d) {
 // LDS bytes of a pass: the tile in the storage type, the per-wave slots and,
 // for complex64 adjoint passes, the per-thread accumulators.
 // tab_bytes: the staged table of the pass (0 if none).
+// dtype: 1 complex64, 0 complex128, 2 complex64 with FP64 LDS tiles.
 __host__ __device__ inline size_t pass_lds_bytes(int dtype, int K, int threads, bool two, int tab_bytes = 0) {
   return (size_t)(1u << K) * (dtype == 1 ? 8 : 16) + (size_t)((threads + 63) / 64) * (kMaxK + 1) * sizeof(double) +
          (two && dtype == 1 ? (size_t)K * threads * sizeof(double) : 0) + (size_t)tab_bytes;
+}
+template <class S>
+constexpr int kernel_dtype() {
+  return std::is_same<S, float>::value ? 1 : std::is_same<S, double>::value ? 0 : 2;
 }
 
 // BIG: the state is 4 GiB or larger, so byte offsets need 64-bit addresses.
@@ -1013,7 +1038,7 @@ __host__ __device__ inline size_t pass_lds_bytes(int dtype, int K, int threads, 
 template <class S, int R, bool TWO, int PH, bool POST>
 hipError_t launch_pass_t(const PassDesc* d, int K, int grid, int batch, int threads, bool big, hipStream_t st,
                          int* occ, int tab_bytes) {
-  const size_t lds = pass_lds_bytes(sizeof(S) == 4 ? 1 : 0, K, threads, TWO, tab_bytes);
+  const size_t lds = pass_lds_bytes(kernel_dtype<S>(), K, threads, TWO, tab_bytes);
   if (occ) {
     if (big)
       return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_pass<S, R, TWO, PH, POST, true>, threads, lds);
@@ -1028,7 +1053,7 @@ hipError_t launch_pass_t(const PassDesc* d, int K, int grid, int batch, int thre
 
 template <class S, int R, bool TWO, int PH, bool POST>
 hipError_t set_attr_t() {
-  const int maxlds = (int)pass_lds_bytes(sizeof(S) == 4 ? 1 : 0, kMaxK, 512, TWO, 16 * kLutMax + 8 * (kMaxQubits + 1));
+  const int maxlds = (int)pass_lds_bytes(kernel_dtype<S>(), kMaxK, 512, TWO, 16 * kLutMax + 8 * (kMaxQubits + 1));
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pass<S, R, TWO, PH, POST, false>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, maxlds);
   if (e == hipSuccess)

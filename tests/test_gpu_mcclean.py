@@ -149,3 +149,55 @@ def test_tune_applies_cached_shape(gpu, tmp_path, monkeypatch):
     Eo, go = sv.mcclean_grad_run_adjoint(n, axes, angles, obs)
     assert abs(E - Eo) <= C128_TOL * max(1.0, abs(Eo))
     assert rel_err(g, go) <= C128_TOL
+
+
+def _lds64(c, mode=None):
+    import ctypes
+    from qradient_amd import _native
+    h = c._get_plan()
+    if mode is not None:
+        _native.call('qg_plan_set_lds64', h, mode)
+    v = ctypes.c_int()
+    _native.call('qg_plan_lds64', h, ctypes.byref(v))
+    return v.value
+
+
+@pytest.mark.parametrize('n,L,k', [(14, 6, 0), (17, 5, 2), (20, 3, 0)])
+def test_fp64_tiles_c64_matches_oracle(gpu, n, L, k):
+    """complex64 passes with FP64 LDS tiles, history and checkpoint modes."""
+    from qradient_amd.circuit_logic import McClean
+    axes, angles, obs = _case(n, L, 90 + n)
+    c = McClean(n, obs, L, axes=axes, angles=angles, dtype='complex64', ckpt_every=k)
+    assert _lds64(c, 1) == 1
+    E, g = c.grad_run()
+    Eo, go = sv.mcclean_grad_run_adjoint(n, axes, angles, obs)
+    assert abs(E - Eo) <= C64_TOL * max(1.0, abs(Eo))
+    assert rel_err(g, go) <= C64_TOL
+
+
+def test_fp64_tiles_chosen_for_long_circuits(gpu):
+    from qradient_amd.circuit_logic import McClean
+    axes, angles, obs = _case(16, 101, 7)
+    long = McClean(16, obs, 101, axes=axes, angles=angles, dtype='complex64')   # 2 passes x 101 layers
+    assert _lds64(long) == 1
+    short = McClean(16, obs, 99, axes=axes[:99], angles=angles[:99], dtype='complex64')
+    assert _lds64(short) == 0
+    c128 = McClean(16, obs, 101, axes=axes, angles=angles)
+    assert _lds64(c128) == 0
+
+
+def test_fp64_tiles_reduce_c64_error(gpu):
+    """On a long circuit the FP64 tiles (one rounding per pass) stay closer to
+    the complex128 oracle than FP32 tiles (a rounding per transpose too)."""
+    from qradient_amd.circuit_logic import McClean
+    n, L = 16, 90
+    axes, angles, obs = _case(n, L, 95)
+    c = McClean(n, obs, L, axes=axes, angles=angles, dtype='complex64')
+    _, go = sv.mcclean_grad_run_adjoint(n, axes, angles, obs)
+    _lds64(c, 0)
+    _, g32 = c.grad_run()
+    _lds64(c, 1)
+    _, g64 = c.grad_run()
+    e32, e64 = rel_err(g32, go), rel_err(g64, go)
+    assert e64 <= C64_TOL and e32 <= C64_TOL
+    assert e64 < e32, (e64, e32)

@@ -124,7 +124,8 @@ int qg_plan_set_profiling(qg_plan* plan, int on);
  * (0 forward pass, 1 reverse two-state pass, 2 forward pass fused with the
  * observable); *count receives the launch count, at most cap are written. */
 int qg_plan_pass_times(const qg_plan* plan, int cap, float* ms, int* kinds, int* count);
-/* Tile shape of a McClean plan: kmax local bits per pass (4..13), at least
+/* Tile shape of a McClean plan: kmax local bits per pass (4..14; 14 only for
+ * QG_C64 plans that keep the per-pass history with FP32 tiles), at least
  * cmin contiguous low bits in high-bit passes, and log2 amplitudes per thread
  * of the forward and adjoint passes (r_fwd = r_rev = 4).  The plan is laid out
  * again (its pass count, memory mode and workspace size may change; a

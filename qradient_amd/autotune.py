@@ -26,8 +26,8 @@ def _passes(n, kmax, cmin):
 
 def default_shape(n, dtype):
     """The library's untuned shape: the fewest passes per layer 13 local bits
-    allow, with the smallest tile (>= 11 bits for complex64, 12 for complex128)
-    that keeps that count."""
+    allow, with the smallest tile (>= 11 bits for complex64, 12 for
+    complex128) that keeps that count."""
     best = _passes(n, 13, 3)
     for kmax in range(11 if dtype == 'complex64' else 12, 13):
         if _passes(n, kmax, 3) == best:
@@ -70,7 +70,7 @@ def cached(kind, n, L, dtype, entangler):
 
 def candidates(dtype):
     out = []
-    for kmax in (13, 12, 11, 10):
+    for kmax in ((14, 13, 12, 11, 10) if dtype == 'complex64' else (13, 12, 11, 10)):
         for cmin in (2, 3, 4, 5, 6):
             rs = (4,)
             for r_fwd in rs:

@@ -226,15 +226,22 @@ static std::vector<Geo> layer_geometry(int n, int kmax, int cmin) {
 // for complex64, 12 for complex128).  Smaller tiles mean smaller workgroups
 // (2^(K-4) threads): more of them are resident per CU, so one workgroup's
 // barrier waits overlap another's work.
+// complex64 passes may take 14 local bits (1024-thread workgroups, phase-free
+// forward and reduce-first adjoint passes only) when that saves a pass per
+// layer (wide = true: the caller can run such passes).  Not the default: one
+// 1024-thread workgroup per CU (128 KiB of LDS) runs the two-state pass at
+// about 40% of the copy bandwidth, so two 14-bit passes take as long as three
+// 11-bit ones; qg_plan_tune and the autotuner can still select it.
 static int passes_per_layer(int n, int K, int cmin) {
   return n <= K ? 1 : 1 + (n - K + (K - cmin) - 1) / (K - cmin);
 }
-static int default_kmax(int n, int dtype, int cmin) {
+static int default_kmax(int n, int dtype, int cmin, bool wide = false) {
   const int kmin = dtype == QG_C64 ? 11 : 12;
-  const int best = passes_per_layer(n, kMaxK, cmin);
-  for (int K = kmin; K < kMaxK; ++K)
+  const int best = passes_per_layer(n, 13, cmin);
+  if (wide && dtype == QG_C64 && n > kFusedMaxN && passes_per_layer(n, kMaxKWide, cmin) < best) return kMaxKWide;
+  for (int K = kmin; K < 13; ++K)
     if (passes_per_layer(n, K, cmin) == best) return K;
-  return kMaxK;
+  return 13;
 }
 
 // Register bits of a pass by role: complex64 forward and history-adjoint
@@ -302,7 +309,8 @@ struct qg_plan {
   int ckpt_req = 0;  // requested period (0: history when it fits)
   int nckpt = 0;
   size_t off_work = 0, off_ckpt = 0;
-  int kmax = kMaxK, cmin = 3;  // cmin 3: contiguous runs of >= 8 amplitudes in high-bit passes
+  int kmax = 13, cmin = 3;  // cmin 3: contiguous runs of >= 8 amplitudes in high-bit passes
+  bool kmax_auto = true;    // kmax from default_kmax (not set by qg_plan_tune)
   int r_fwd = 4, r_rev = 4;  // amplitudes per thread (log2) of forward / history-adjoint passes (complex64)
   int lds64_req = -1;  // complex64 McClean: FP64 LDS tiles (-1 auto, 0 no, 1 yes)
   bool lds64 = false;
@@ -348,6 +356,8 @@ struct qg_plan {
   std::vector<float> pass_ms;  // durations of the last profiled run
   std::vector<int> pass_kind;  // 0 forward, 1 reverse (two-state), 2 forward+observable
   int last_state = -1;  // 0: final state in hist, used by get_state
+  int obs_desc = -1;     // McClean: descriptor whose row K holds E
+  int adj_desc = -1;     // McClean: first adjoint descriptor
   bool state_sign = false;  // the pass kernels' final state is -1 times the circuit's
   void* last_state_ptr = nullptr;
 };
@@ -395,11 +405,20 @@ static int grid_for_pass(const qg_plan* p, const Geo& g, bool two, bool post, in
   return (int)grid;
 }
 
+// Geometry of the gate-free pass that forms lambda = D psi and E when the
+// last forward pass cannot carry it (a 1024-thread pass).
+static Geo obs_geo(int n) {
+  const int K = std::min(n, 11);
+  return make_geo(K, K, n, 0, 0);
+}
+
 // The largest grid any pass variant of these geometries may use (the stride
 // of the partial rows).
 static int grid_max(const qg_plan* p) {
   int gm = 1;
-  for (const Geo& g : p->geo)
+  std::vector<Geo> geos = p->geo;
+  if (p->kind == PK_MCCLEAN) geos.push_back(obs_geo(p->n));  // the observable's own pass
+  for (const Geo& g : geos)
     for (int role = 0; role < 3; ++role)
       for (int phase = 0; phase < 3; ++phase) {
         bool tf;
@@ -582,6 +601,8 @@ static int run_launches(qg_plan* p, int ndesc) {
   qg_ctx* c = p->ctx;
   for (Launch& l : p->launches)
     if (l.kind == 0) {
+      // partial rows are strided by gmax: a larger grid would overlap them
+      if (l.grid > p->gmax) return fail(QG_ERR_HIP, "internal: pass grid exceeds the partial-row stride");
       const PassDesc& d = p->h_desc[l.desc];
       // passes without a phase use the kernel variant without phase code
       if (!(d.flags & (F_PHASE_PRE | F_PHASE_POST))) l.phase = 0;
@@ -688,8 +709,24 @@ static void free_small(qg_plan* p) {
 // and the small device tables of a McClean plan.
 static int mcclean_layout(qg_plan* p) {
   const int n = p->n, L = p->L;
+  if (p->kmax_auto) p->kmax = default_kmax(n, p->dtype, p->cmin);
   p->geo = layer_geometry(n, p->kmax, p->cmin);
-  const int P = (int)p->geo.size();
+  int P = (int)p->geo.size();
+  if (p->kmax > 13) {
+    // 14 local bits run only as complex64 history passes with FP32 tiles
+    // (1024-thread variants exist for those); otherwise fall back to 13
+    const size_t S = (size_t)(1ull << n) * 8;
+    size_t fb = 0, tb = 0;
+    hipMemGetInfo(&fb, &tb);
+    if (p->ws_owned && p->ws) fb += p->ws_bytes;
+    const bool hist_fits = p->ckpt_req <= 0 && (size_t)(P * L + 4) * S <= (size_t)(0.85 * (double)fb);
+    const bool fp64_tiles = p->lds64_req == 1 || (p->lds64_req < 0 && P * L > 200);
+    if (p->dtype != QG_C64 || !hist_fits || fp64_tiles) {
+      p->kmax = p->kmax_auto ? default_kmax(n, p->dtype, p->cmin, false) : 13;
+      p->geo = layer_geometry(n, p->kmax, p->cmin);
+      P = (int)p->geo.size();
+    }
+  }
   p->S = (size_t)(1ull << n) * (p->dtype == QG_C64 ? 8 : 16);
   // complex64 states are rounded at every LDS transpose of a pass with FP32
   // tiles; beyond about 200 passes these roundings would take the gradient
@@ -732,7 +769,7 @@ static int mcclean_layout(qg_plan* p) {
   p->off_prep = p->off_ini + p->S;
   p->ws_bytes = p->off_prep + p->S;
   p->gmax = grid_max(p);
-  const int npass = 2 * P * L + 2 * P;  // forward + reverse + ini-state preparation
+  const int npass = 2 * P * L + 2 * P + 1;  // forward + reverse + ini-state preparation + observable
   p->nrows = (P * L + 1) * (kMaxK + 1);
   CHK(alloc_small(p, npass, p->nrows, 1 + L * n));
   // static tables
@@ -763,7 +800,6 @@ extern "C" int qg_mcclean_plan(qg_ctx* ctx, int n, int L, int dtype, int entangl
   p->dtype = dtype;
   p->ent = entangler;
   p->ckpt_req = ckpt_every;
-  p->kmax = default_kmax(n, dtype, p->cmin);
   int r = mcclean_layout(p);
   if (r != QG_OK) {
     qg_plan_destroy(p);
@@ -780,6 +816,7 @@ static int mcclean_schedule(qg_plan* p, const int32_t* axes, const double* angle
   const int n = p->n, L = p->L, P = (int)p->geo.size();
   const bool diag_obs = obs.xy.empty();
   p->launches.clear();
+  p->obs_desc = p->adj_desc = -1;
   int nd = 0;
   int row = 0;
   auto hist = [&](int i, int g) { return wsp(p, p->off_hist + ((size_t)i * P + g) * p->S); };
@@ -849,7 +886,10 @@ static int mcclean_schedule(qg_plan* p, const int32_t* axes, const double* angle
     for (int g = 0; g < P; ++g) {
       const Geo& G = p->geo[g];
       const bool last = (i == L - 1) && (g == P - 1);
-      const bool obs_fused = last && diag_obs;
+      // the observable rides on the last forward pass, except after a
+      // 1024-thread pass (no such variant of the general two-state form)
+      const bool obs_fused = last && diag_obs && G.K <= 13;
+      if (obs_fused) p->obs_desc = nd;
       PassDesc& d = push(G, obs_fused, obs_fused ? ROLE_TWO_FP64 : ROLE_FWD);
       uint32_t f = F_ROT_A | F_STORE_A;
       if (g == 0) {
@@ -881,9 +921,21 @@ static int mcclean_schedule(qg_plan* p, const int32_t* axes, const double* angle
       cur = out;
     }
   }
+  if (p->obs_desc < 0 && diag_obs) {
+    // lambda = D psi and E in a gate-free pass of its own
+    const Geo og = obs_geo(n);
+    p->obs_desc = nd;
+    PassDesc& d = push(og, true, ROLE_TWO_FP64);
+    d.flags = F_LOAD_A | F_OBS_DIAG | F_STORE_B;
+    d.a_in = cur;
+    d.b_out = lam[0];
+    d.row_base = row;
+    row += kMaxK + 1;
+  }
   p->last_state_ptr = (void*)cur;
   p->state_sign = false;
   for (int k = 0; k < L * n; ++k) p->state_sign ^= gate_flips(angles[k], 1.0);
+  p->adj_desc = nd;
   if (!grad) {
     ndesc_out = nd;
     return QG_OK;
@@ -1056,7 +1108,7 @@ static int mcclean_run(qg_plan* p, const int32_t* axes, const double* angles, co
     return fail(QG_ERR_INVALID, "too many observable terms");
   qg_ctx* c = p->ctx;
   HIPCHK(hipSetDevice(c->device));
-  if (p->fused && n <= kMaxK && !ini_state) {
+  if (p->fused && n <= kFusedMaxN && !ini_state) {
     // the whole circuit in one workgroup (qg_fused.hip): gates as (type, cos, sin)
     std::vector<double> res(1 + (size_t)L * n);
     auto fill = [&](int, FusedDescH& d, Arena& ar) -> int {
@@ -1111,12 +1163,11 @@ static int mcclean_run(qg_plan* p, const int32_t* axes, const double* angles, co
     for (int k = 0; k <= kMaxK; ++k) cnt[d.row_base + k] = l.grid;
   }
   {
-    // E: the row K of the fused observable pass (the last forward pass)
-    int fwd_last = (ini_state ? P : 0) + P * L - 1;
-    const PassDesc& d = p->h_desc[fwd_last];
+    // E: row K of the observable pass
+    const PassDesc& d = p->h_desc[p->obs_desc];
     out_rows[0].push_back(d.row_base + d.K);
     if (grad) {
-      int idx = fwd_last + 1;
+      int idx = p->adj_desc;
       for (int i = L - 1; i >= 0; --i) {
         for (int g = P - 1; g >= 0; --g, ++idx) {
           const PassDesc& dr = p->h_desc[idx];
@@ -1241,7 +1292,7 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
   const int Wi = (int)W;
   qg_ctx* c = p->ctx;
   HIPCHK(hipSetDevice(c->device));
-  if (p->fused && n <= kMaxK) {
+  if (p->fused && n <= kFusedMaxN) {
     const int per = 1 + 2 * L;
     std::vector<double> res((size_t)B * per);
     auto fill = [&](int b, FusedDescH& d, Arena& ar) -> int {
@@ -1755,11 +1806,13 @@ extern "C" int qg_plan_lds64(const qg_plan* p, int* out) {
 
 extern "C" int qg_plan_tune(qg_plan* p, int kmax, int cmin, int r_fwd, int r_rev) {
   if (!p) return fail(QG_ERR_INVALID, "NULL plan");
-  if (kmax < 4 || kmax > kMaxK) return fail(QG_ERR_INVALID, "kmax must be in [4, 13]");
+  if (kmax < 4 || kmax > kMaxK) return fail(QG_ERR_INVALID, "kmax must be in [4, 14]");
+  if (kmax > 13 && p->dtype != QG_C64) return fail(QG_ERR_INVALID, "kmax 14 needs a QG_C64 plan");
   if (cmin < 0 || cmin >= kmax) return fail(QG_ERR_INVALID, "cmin must be in [0, kmax)");
   if (r_fwd != 4 || r_rev != 4) return fail(QG_ERR_INVALID, "r_fwd, r_rev must be 4 (16 amplitudes per thread)");
   if (p->kind != PK_MCCLEAN) return fail(QG_ERR_UNSUPPORTED, "tuning applies to McClean plans");
   p->kmax = kmax;
+  p->kmax_auto = false;
   p->cmin = cmin;
   p->r_fwd = r_fwd;
   p->r_rev = r_rev;

@@ -289,7 +289,7 @@ hipError_t launch_fused(const void* descs, int batch, int n, hipStream_t st) {
 }
 
 hipError_t set_fused_attributes() {
-  const int maxlds = (int)fused_lds_bytes(kMaxK);
+  const int maxlds = (int)fused_lds_bytes(kFusedMaxN);
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused<4>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, maxlds);
   return e;

@@ -8,7 +8,9 @@
 
 namespace qg {
 
-constexpr int kMaxK = 13;        // max local (tile) bits of a pass: 2^13 FP64 complex = 128 KiB LDS
+constexpr int kMaxK = 14;        // max local (tile) bits of a pass (14: complex64 only, 2^14 x 8 B = 128 KiB LDS)
+constexpr int kMaxKWide = 14;    // local bits that need 1024-thread workgroups (complex64 history passes)
+constexpr int kFusedMaxN = 13;   // largest circuit run whole in one workgroup (2^13 FP64 complex = 128 KiB LDS)
 constexpr int kRegBits = 4;      // amplitudes held per thread = 2^kRegBits
 constexpr int kMaxStages = 4;    // natural stage + ceil((kMaxK - kRegBits) / kRegBits)
 constexpr int kMaxQubits = 32;     // 32-bit amplitude indices

@@ -603,14 +603,17 @@ __device__ __forceinline__ void tile_addresses(const PassDesc* __restrict__ d, b
 // end; complex128 adjoint passes reduce per tile into per-wave slots.  Both
 // use a fixed order, so results are bitwise reproducible.
 // ---------------------------------------------------------------------------
-template <class S, int R, bool TWO, int PH, bool POST, bool BIG>
+template <class S, int R, bool TWO, int PH, bool POST, bool BIG, bool WIDE = false>
 __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* lds, uint32_t tid_in, uint32_t blk,
                                          uint32_t nblk) {
   constexpr int A = 1 << R;
   typedef typename Store<S>::G G;  // HBM element type
   typedef typename Store<S>::L LT; // LDS tile element type
-  constexpr bool PF = !(TWO && (sizeof(G) == 8 || POST || PH != 0));
-  constexpr bool ACCL = TWO && sizeof(LT) == 4;
+  // WIDE: a 1024-thread workgroup (14 local bits), which must fit 128 VGPRs
+  // and leaves no LDS for per-thread accumulators: no prefetch in adjoint
+  // passes, per-tile wave sums
+  constexpr bool PF = !(TWO && (sizeof(G) == 8 || POST || PH != 0 || WIDE));
+  constexpr bool ACCL = TWO && sizeof(LT) == 4 && !WIDE;
   // diagonal ops after the gates: adjoint passes of the general form, and
   // forward passes with a phase (which may sit before or after the gates)
   constexpr bool POST_DIAG = POST || (!TWO && PH != 0);
@@ -1018,14 +1021,28 @@ __global__ __launch_bounds__(512) void k_pass(const PassDesc* __restrict__ d) {
   extern __shared__ __align__(16) char lds_dyn[];
   run_pass<S, R, TWO, PH, POST, BIG>(d + blockIdx.y, lds_dyn, threadIdx.x, blockIdx.x, gridDim.x);
 }
+// 1024-thread workgroups: 14 local bits with 16 amplitudes per thread.
+template <class S, int R, bool TWO, int PH, bool POST, bool BIG>
+__global__ __launch_bounds__(1024) void k_pass_w(const PassDesc* __restrict__ d) {
+  extern __shared__ __align__(16) char lds_dyn[];
+  run_pass<S, R, TWO, PH, POST, BIG, true>(d + blockIdx.y, lds_dyn, threadIdx.x, blockIdx.x, gridDim.x);
+}
+// Variants built with 1024-thread workgroups: complex64 (FP32 tiles),
+// 16 amplitudes per thread, phase-free forward and reduce-first adjoint passes.
+template <class S, int R, bool TWO, int PH, bool POST>
+constexpr bool has_wide() {
+  return std::is_same<S, float>::value && R == 4 && PH == 0 && !POST;
+}
 
 // LDS bytes of a pass: the tile in the storage type, the per-wave slots and,
 // for complex64 adjoint passes, the per-thread accumulators.
 // tab_bytes: the staged table of the pass (0 if none).
 // dtype: 1 complex64, 0 complex128, 2 complex64 with FP64 LDS tiles.
-__host__ __device__ inline size_t pass_lds_bytes(int dtype, int K, int threads, bool two, int tab_bytes = 0) {
+// wide: a 1024-thread variant (no per-thread accumulators).
+__host__ __device__ inline size_t pass_lds_bytes(int dtype, int K, int threads, bool two, int tab_bytes = 0,
+                                                 bool wide = false) {
   return (size_t)(1u << K) * (dtype == 1 ? 8 : 16) + (size_t)((threads + 63) / 64) * (kMaxK + 1) * sizeof(double) +
-         (two && dtype == 1 ? (size_t)K * threads * sizeof(double) : 0) + (size_t)tab_bytes;
+         (two && dtype == 1 && !wide ? (size_t)K * threads * sizeof(double) : 0) + (size_t)tab_bytes;
 }
 template <class S>
 constexpr int kernel_dtype() {
@@ -1038,7 +1055,22 @@ constexpr int kernel_dtype() {
 template <class S, int R, bool TWO, int PH, bool POST>
 hipError_t launch_pass_t(const PassDesc* d, int K, int grid, int batch, int threads, bool big, hipStream_t st,
                          int* occ, int tab_bytes) {
-  const size_t lds = pass_lds_bytes(kernel_dtype<S>(), K, threads, TWO, tab_bytes);
+  const size_t lds = pass_lds_bytes(kernel_dtype<S>(), K, threads, TWO, tab_bytes, threads > 512);
+  if (threads > 512) {
+    if constexpr (has_wide<S, R, TWO, PH, POST>()) {
+      if (occ) {
+        if (big)
+          return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_pass_w<S, R, TWO, PH, POST, true>, threads, lds);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_pass_w<S, R, TWO, PH, POST, false>, threads, lds);
+      }
+      if (big)
+        hipLaunchKernelGGL((k_pass_w<S, R, TWO, PH, POST, true>), dim3(grid, batch), dim3(threads), lds, st, d);
+      else
+        hipLaunchKernelGGL((k_pass_w<S, R, TWO, PH, POST, false>), dim3(grid, batch), dim3(threads), lds, st, d);
+      return hipGetLastError();
+    }
+    return hipErrorInvalidValue;
+  }
   if (occ) {
     if (big)
       return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_pass<S, R, TWO, PH, POST, true>, threads, lds);
@@ -1053,12 +1085,23 @@ hipError_t launch_pass_t(const PassDesc* d, int K, int grid, int batch, int thre
 
 template <class S, int R, bool TWO, int PH, bool POST>
 hipError_t set_attr_t() {
-  const int maxlds = (int)pass_lds_bytes(kernel_dtype<S>(), kMaxK, 512, TWO, 16 * kLutMax + 8 * (kMaxQubits + 1));
+  // the largest tile a 512-thread variant runs (13 local bits), and the LDS
+  // of a 1024-thread variant (14 local bits)
+  const int maxlds = (int)pass_lds_bytes(kernel_dtype<S>(), 13, 512, TWO, 16 * kLutMax + 8 * (kMaxQubits + 1));
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pass<S, R, TWO, PH, POST, false>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, maxlds);
   if (e == hipSuccess)
     e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pass<S, R, TWO, PH, POST, true>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, maxlds);
+  if constexpr (has_wide<S, R, TWO, PH, POST>()) {
+    const int wlds = (int)pass_lds_bytes(kernel_dtype<S>(), kMaxKWide, 1024, TWO, 8 * (kMaxQubits + 1), true);
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pass_w<S, R, TWO, PH, POST, false>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, wlds);
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pass_w<S, R, TWO, PH, POST, true>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, wlds);
+  }
   return e;
 }
 

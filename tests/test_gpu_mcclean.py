@@ -201,3 +201,26 @@ def test_fp64_tiles_reduce_c64_error(gpu):
     e32, e64 = rel_err(g32, go), rel_err(g64, go)
     assert e64 <= C64_TOL and e32 <= C64_TOL
     assert e64 < e32, (e64, e32)
+
+
+@pytest.mark.parametrize('n,L,entangler', [(15, 4, 'cnot'), (18, 3, 'cz'), (20, 3, 'cnot')])
+def test_wide_tiles_c64_matches_oracle(gpu, n, L, entangler):
+    """14 local bits per pass (1024-thread workgroups) for complex64 history
+    plans; the observable then takes a gate-free pass of its own."""
+    import ctypes
+    from qradient_amd import _native
+    from qradient_amd.circuit_logic import McClean
+    axes, angles, obs = _case(n, L, 110 + n)
+    c = McClean(n, obs, L, axes=axes, angles=angles, dtype='complex64', entangler=entangler)
+    _native.call('qg_plan_tune', c._get_plan(), 14, 3, 4, 4)
+    E, g = c.grad_run()
+    Eo, go = sv.mcclean_grad_run_adjoint(n, axes, angles, obs, entangler=entangler)
+    assert abs(E - Eo) <= C64_TOL * max(1.0, abs(Eo))
+    assert rel_err(g, go) <= C64_TOL
+    E2, g2 = c.grad_run()
+    assert E2 == E and np.array_equal(g2, g)
+    assert abs(c.run_expec_val() - Eo) <= C64_TOL * max(1.0, abs(Eo))
+    c.grad_run()
+    psi = c.final_state()
+    ref = sv.mcclean_forward(n, axes, angles, entangler=entangler)
+    assert np.max(np.abs(psi - ref)) <= 1e-5

@@ -590,6 +590,16 @@ static void desc_slots(PassDesc& d, int R, bool two, bool red_first) {
   }
   // A's last gate matters only if A is used after the gates
   if (!a_used_after && last_a >= 0) d.sflag[last_a] &= ~SL_ROT_A;
+  // owned thread bits of a reduce-first pass
+  d.nxy = 0;
+  d.zmask = 0;
+  if (two && red_first && (d.flags & F_REDUCE) && d.row_base >= 0) {
+    for (int lb = 0; lb < d.t; ++lb) {
+      if (!d.own[lb] || d.gtype[lb] == G_NONE) continue;
+      if (d.gtype[lb] == G_Z) d.zmask |= 1u << lb;
+      else d.xy[d.nxy++] = lb | (d.gtype[lb] << 8);
+    }
+  }
   // stages after the last one with work (only spectator bits) are skipped
   d.nstage = 1;
   for (int s = 0; s < ns; ++s)

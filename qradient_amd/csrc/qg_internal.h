@@ -79,6 +79,11 @@ struct PassDesc {
   int32_t sflag[kMaxSlots];
   int32_t slb[kMaxSlots];  // local bit of the slot
   int32_t nstage;          // stages that hold a gate or a reduction (later stages are skipped)
+  // reduce-first adjoint passes: the owned thread bits (local bits below t),
+  // X / Y ones as a list (bit | type << 8) and Z ones as a mask
+  int32_t nxy;
+  int32_t xy[kMaxK];
+  uint32_t zmask;
   const uint32_t* map;   // ladder map columns (n entries), or null
   const double* init_tab;// n + 1 entries when F_INIT_TAB
   const DTerm* dterms;   // diagonal quadratic form

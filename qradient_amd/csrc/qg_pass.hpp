@@ -619,10 +619,11 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
   constexpr bool POST_DIAG = POST || (!TWO && PH != 0);
   // the workgroup's FP64 sum of the diagonal reductions (d/dgamma, E)
   constexpr bool DSUM = TWO && (POST || PH != 0);
-  // Register type of A.  A reduce-first complex64 pass only reads A (the
-  // FP32 pass output) in its FP64 reductions, so it stays in FP32 registers
-  // (exact, or rounded once after a pre phase); everywhere else A is FP64.
-  typedef typename std::conditional<TWO && !POST && sizeof(G) == 4, float, double>::type TAr;
+  // Register type of A: FP64, except in a reduce-first complex64 pass with a
+  // phase or with 1024 threads, which only reads A in its FP64 reductions and
+  // keeps it in FP32 registers (exact, or rounded once after the phase) to
+  // stay within the register file.
+  typedef typename std::conditional<TWO && !POST && (PH != 0 || WIDE) && sizeof(G) == 4, float, double>::type TAr;
   const int n = d->n, K = d->K, t = d->t, c = d->c, p = d->p;
   const uint32_t flags = d->flags;
   const uint32_t ntiles = 1u << (n - K);
@@ -827,36 +828,68 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
         for (int a = 0; a < A; ++a)
           Stor<LT>::lds_st_b(lds, swz<LT>(tid | ((uint32_t)a << t)) << SH, xa[a]);
         __syncthreads();
-#pragma unroll 1
-        for (int lb = 0; lb < t; ++lb) {
-          if (!ldc(d->own, lb)) continue;
-          const int ty = ldc(d->gtype, lb);
-          const double sg = ((tid >> lb) & 1u) ? -1.0 : 1.0;  // +1 where the bit is 0
-          double v;
-          if (ty == G_Z) {
-            v = sg * msum;
-          } else {
-            const uint32_t pb = swz<LT>(tid ^ (1u << lb)) << SH;
-            double v0 = 0.0, v1 = 0.0;
-#pragma unroll
-            for (int a = 0; a < A; ++a) {
-              const Cd pa = Stor<LT>::template lds_ld_b<double>(lds, pb ^ (swz<LT>((uint32_t)a << t) << SH));
-              double& acc = (a & 1) ? v1 : v0;
-              if (ty == G_X) {  // Im(conj(B_x) A_{x^q})
-                acc = fmd(xb[a].re, pa.im, acc);
-                acc = fmd(-xb[a].im, pa.re, acc);
-              } else {          // -sg * Re(conj(B_x) A_{x^q})
-                acc = fmd(xb[a].re, pa.re, acc);
-                acc = fmd(xb[a].im, pa.im, acc);
-              }
-            }
-            v = (ty == G_X) ? v0 + v1 : -sg * (v0 + v1);
-          }
+        auto accum = [&](int lb, double v) __attribute__((always_inline)) {
           if constexpr (ACCL) {
             tacc[lb * nthr + tid] += v;
           } else {
             const double w = wave_sum(v, width);
             if ((tid & 63) == 0) wacc[(tid >> 6) * (K + 1) + lb] += w;
+          }
+        };
+        // Z on a thread bit: +-(the thread's sum), by the bit of the thread
+        for (uint32_t zm = ldc(&d->zmask, 0); zm; zm &= zm - 1) {
+          const int lb = __builtin_ctz(zm);
+          accum(lb, ((tid >> lb) & 1u) ? -msum : msum);
+        }
+        // X / Y: the partner amplitudes of A (thread tid ^ 2^lb) from LDS,
+        // raw; with FP32 tiles the next bit's reads are in flight while the
+        // current bit is reduced
+        typedef Cx<LT> Raw[A];
+        auto rd = [&](int e, Raw& P) __attribute__((always_inline)) {
+          const uint32_t pb = swz<LT>(tid ^ (1u << (e & 255))) << SH;
+#pragma unroll
+          for (int a = 0; a < A; ++a)
+            P[a] = Stor<LT>::template lds_ld_b<LT>(lds, pb ^ (swz<LT>((uint32_t)a << t) << SH));
+        };
+        auto use = [&](int e, const Raw& P) __attribute__((always_inline)) {
+          const int lb = e & 255;
+          const bool isx = (e >> 8) == G_X;
+          double v0 = 0.0, v1 = 0.0;
+#pragma unroll
+          for (int a = 0; a < A; ++a) {
+            const double pr = (double)P[a].re, pi = (double)P[a].im;
+            double& acc = (a & 1) ? v1 : v0;
+            if (isx) {  // Im(conj(B_x) A_{x^q})
+              acc = fmd(xb[a].re, pi, acc);
+              acc = fmd(-xb[a].im, pr, acc);
+            } else {    // Re(conj(B_x) A_{x^q}), signed below
+              acc = fmd(xb[a].re, pr, acc);
+              acc = fmd(xb[a].im, pi, acc);
+            }
+          }
+          const double v = v0 + v1;
+          // Y: -sg * Re(...), sg = +1 where the bit is 0
+          accum(lb, isx ? v : (((tid >> lb) & 1u) ? v : -v));
+        };
+        const int nxy = ldc(&d->nxy, 0);
+        if constexpr (sizeof(LT) == 4 && !WIDE) {
+          Raw P0, P1;
+          if (nxy > 0) rd(ldc(d->xy, 0), P0);
+          int i = 0;
+#pragma unroll 1
+          for (; i + 1 < nxy; i += 2) {
+            rd(ldc(d->xy, i + 1), P1);
+            use(ldc(d->xy, i), P0);
+            if (i + 2 < nxy) rd(ldc(d->xy, i + 2), P0);
+            use(ldc(d->xy, i + 1), P1);
+          }
+          if (i < nxy) use(ldc(d->xy, i), P0);
+        } else {
+          Raw P0;
+#pragma unroll 1
+          for (int i = 0; i < nxy; ++i) {
+            rd(ldc(d->xy, i), P0);
+            use(ldc(d->xy, i), P0);
           }
         }
         __syncthreads();  // the tile buffer is B's from here on

@@ -136,6 +136,14 @@ int qg_plan_tune(qg_plan* plan, int kmax, int cmin, int r_fwd, int r_rev);
 /* on = 1 (default): circuits with n <= kmax run whole in one workgroup each
  * (one launch per call, FP64 throughout); on = 0 uses the pass kernels. */
 int qg_plan_set_fused(qg_plan* plan, int on);
+/* on = 1: a run's pass launches are captured once as a HIP graph and
+ * replayed while the plan's launch list is unchanged (a run uploads its angles
+ * and tables to the plan's descriptor table before the replay); on = 0
+ * (default) launches every pass directly.  Profiled runs launch directly.
+ * Enable it only where no other thread makes device-synchronising HIP calls
+ * (allocation, synchronous copies) while this plan runs: those invalidate a
+ * capture in progress. */
+int qg_plan_set_graph(qg_plan* plan, int on);
 /* LDS tile precision of a QG_C64 McClean plan: mode 1 keeps the pass tiles
  * in FP64 (the state is rounded to complex64 once per pass, when stored),
  * 0 in FP32 (also rounded at each of a pass's LDS transposes; faster), -1

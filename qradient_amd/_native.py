@@ -59,6 +59,7 @@ SIGNATURES = {
     'qg_plan_set_fused': [_vp, _i],
     'qg_plan_ckpt_every': [_vp, _ip],
     'qg_plan_set_lds64': [_vp, _i],
+    'qg_plan_set_graph': [_vp, _i],
     'qg_plan_lds64': [_vp, _ip],
     'qg_plan_destroy': [_vp],
     'qg_state_create': [_vp, _i, _i, ctypes.c_char, _pp],

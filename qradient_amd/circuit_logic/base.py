@@ -6,7 +6,7 @@ import numpy as np
 from .. import _native
 from ..physical_components import Observable, State
 
-_BACKEND_KEYS = ('dtype', 'entangler', 'device', 'ckpt_every', 'tune')
+_BACKEND_KEYS = ('dtype', 'entangler', 'device', 'ckpt_every', 'tune', 'graph')
 
 
 def progbar_range(hide_progbar):
@@ -36,6 +36,7 @@ class ParametrizedCircuit:
         self.device = int(backend.get('device', 0))
         self.ckpt_every = int(backend.get('ckpt_every', 0))
         self.tune = bool(backend.get('tune', False))
+        self.graph = bool(backend.get('graph', False))
         self.observable = Observable(self.qnum, observable, store_components=use_observable_components)
         self._state = None
         self._gates = None

@@ -10,7 +10,8 @@ method, FP64 throughout) or ``'complex64'`` (complex64 state, FP64 arithmetic
 and reductions); ``entangler='cnot'`` (reference) or ``'cz'``; ``device``;
 ``ckpt_every``; ``tune=True`` (apply the autotuner's cached tile shape for
 this circuit shape, timing the candidates on the device first if none is
-cached).
+cached); ``graph=True`` (replay the pass launches as a HIP graph; for callers
+that make no concurrent device-synchronising HIP calls from other threads).
 """
 import ctypes
 
@@ -45,6 +46,8 @@ class McClean(ParametrizedCircuit):
                          _native.DTYPES[self.dtype], _native.ENTANGLERS[self.entangler], self.ckpt_every,
                          ctypes.byref(h))
             self._plan = h
+            if self.graph:
+                _native.call('qg_plan_set_graph', h, 1)
             if self.tune:
                 from .. import autotune
                 shape = autotune.cached('mcclean', self.qnum, self.lnum, self.dtype, self.entangler)

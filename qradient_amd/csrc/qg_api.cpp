@@ -356,6 +356,14 @@ struct qg_plan {
   std::vector<float> pass_ms;  // durations of the last profiled run
   std::vector<int> pass_kind;  // 0 forward, 1 reverse (two-state), 2 forward+observable
   int last_state = -1;  // 0: final state in hist, used by get_state
+  // the pass launches of a run as a HIP graph, replayed while the launch list
+  // (variants, grids, descriptor slots) is unchanged; the kernels read their
+  // per-call parameters from the descriptor table, uploaded before each replay.
+  // Off by default: a capture is invalidated by device-synchronising calls
+  // (allocation, synchronous copies) that other threads make meanwhile
+  bool use_graph = false;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<int64_t> gsig;
   int obs_desc = -1;     // McClean: descriptor whose row K holds E
   int adj_desc = -1;     // McClean: first adjoint descriptor
   bool state_sign = false;  // the pass kernels' final state is -1 times the circuit's
@@ -364,6 +372,7 @@ struct qg_plan {
 
 static void* wsp(qg_plan* p, size_t off) { return p->ws + off; }
 static void free_small(qg_plan* p);
+static void drop_graph(qg_plan* p);
 
 static int ensure_workspace(qg_plan* p) {
   if (p->ws) return QG_OK;
@@ -457,6 +466,7 @@ extern "C" int qg_plan_destroy(qg_plan* p) {
   hipStreamSynchronize(p->ctx->stream);
   if (p->ws_owned && p->ws) hipFree(p->ws);
   free_small(p);
+  drop_graph(p);
   for (hipEvent_t e : p->pev) hipEventDestroy(e);
   delete p;
   return QG_OK;
@@ -607,6 +617,23 @@ static void desc_slots(PassDesc& d, int R, bool two, bool red_first) {
       if (d.stype[s * R + kk] != G_NONE || (d.sflag[s * R + kk] & SL_RED)) d.nstage = s + 1;
 }
 
+static void drop_graph(qg_plan* p) {
+  if (p->gexec) hipGraphExecDestroy(p->gexec);
+  p->gexec = nullptr;
+  p->gsig.clear();
+}
+
+static hipError_t launch_all(qg_plan* p) {
+  qg_ctx* c = p->ctx;
+  for (const Launch& l : p->launches) {
+    if (l.kind != 0) continue;
+    const hipError_t e = launch_pass(kdtype(p), l.R, l.two, l.phase, l.post, p->d_desc + l.desc, l.K, l.grid,
+                                     l.batch, l.threads, p->n, c->stream, nullptr, l.tab);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 static int run_launches(qg_plan* p, int ndesc) {
   qg_ctx* c = p->ctx;
   for (Launch& l : p->launches)
@@ -628,6 +655,41 @@ static int run_launches(qg_plan* p, int ndesc) {
     }
   HIPCHK(hipMemcpyAsync(p->d_desc, p->h_desc, sizeof(PassDesc) * ndesc, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipEventRecord(c->ev0, c->stream));
+  if (p->use_graph && !p->profile) {
+    std::vector<int64_t> sig;
+    sig.reserve(p->launches.size() * 11 + 2);
+    sig.push_back(kdtype(p));
+    sig.push_back(p->n);
+    sig.push_back((int64_t)(intptr_t)p->d_desc);
+    for (const Launch& l : p->launches) {
+      const int64_t v[11] = {l.kind, l.R, l.two, l.phase, l.post, l.desc, l.K, l.grid, l.batch, l.threads, l.tab};
+      sig.insert(sig.end(), v, v + 11);
+    }
+    if (!p->gexec || sig != p->gsig) {
+      drop_graph(p);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+      if (e == hipSuccess) {
+        e = launch_all(p);
+        const hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+        if (e == hipSuccess) e = e2;
+      }
+      if (e == hipSuccess) e = hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0);
+      if (g) hipGraphDestroy(g);
+      if (e == hipSuccess) {
+        p->gsig.swap(sig);
+      } else {
+        // capture unavailable: launch directly from now on
+        (void)hipGetLastError();
+        p->gexec = nullptr;
+        p->use_graph = false;
+      }
+    }
+    if (p->gexec) {
+      HIPCHK(hipGraphLaunch(p->gexec, c->stream));
+      return QG_OK;
+    }
+  }
   const size_t nl = p->launches.size();
   if (p->profile) {
     while (p->pev.size() < nl + 1) {
@@ -683,6 +745,7 @@ extern "C" int qg_plan_pass_times(const qg_plan* p, int cap, float* ms, int* kin
 // McClean
 // ---------------------------------------------------------------------------
 static void free_small(qg_plan* p) {
+  drop_graph(p);  // its launches point into the descriptor table freed here
   hipFree(p->d_map);
   hipFree(p->d_init);
   hipFree(p->d_terms);
@@ -1797,6 +1860,15 @@ static int relayout(qg_plan* p) {
     p->ws_owned = false;
   }
   p->last_state_ptr = nullptr;
+  return QG_OK;
+}
+
+extern "C" int qg_plan_set_graph(qg_plan* p, int on) {
+  if (!p) return fail(QG_ERR_INVALID, "NULL plan");
+  HIPCHK(hipSetDevice(p->ctx->device));
+  HIPCHK(hipStreamSynchronize(p->ctx->stream));
+  drop_graph(p);
+  p->use_graph = on != 0;
   return QG_OK;
 }
 

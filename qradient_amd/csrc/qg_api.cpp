@@ -331,6 +331,10 @@ struct qg_plan {
   uint32_t* d_map = nullptr;    // n columns (gather/scatter)
   double* d_init = nullptr;     // n + 1
   DTerm* d_terms = nullptr;     // diagonal terms (max_terms)
+  std::vector<DTerm> h_terms;   // host copy of what d_terms holds for the current run
+  std::vector<DTerm> h_cterms;  // the terms split per pass launch (PassDesc::cterms)
+  DTerm* d_cterms = nullptr;
+  size_t cterm_cap = 0;
   DTerm* d_xy = nullptr;
   int max_terms = 0;
   double* d_lut = nullptr;
@@ -617,6 +621,46 @@ static void desc_slots(PassDesc& d, int R, bool two, bool red_first) {
       if (d.stype[s * R + kk] != G_NONE || (d.sflag[s * R + kk] & SL_RED)) d.nstage = s + 1;
 }
 
+// Splits the diagonal terms of a descriptor by its launch's register bits
+// (PassDesc::cterms, nct, nht, pwc); appends the lists to out and returns
+// their offset there.
+static size_t split_terms(const qg_plan* p, PassDesc& d, int R, std::vector<DTerm>& out) {
+  const size_t off0 = out.size();
+  const size_t base = (size_t)(d.dterms - p->d_terms);
+  int rpos[kMaxR];
+  for (int k = 0; k < R; ++k) {
+    const int l = d.t + k;
+    rpos[k] = l < d.c ? l : d.p + (l - d.c);
+  }
+  auto regk = [&](int b) {
+    for (int k = 0; k < R; ++k)
+      if (rpos[k] == b) return k;
+    return -1;
+  };
+  auto sgn = [](int a, int k) { return ((a >> k) & 1) ? -1.0 : 1.0; };
+  std::vector<DTerm> hl[kMaxR];
+  for (int a = 0; a < (1 << kMaxR); ++a) d.pwc[a] = 0.0;
+  for (int i = 0; i < d.ndterms; ++i) {
+    const DTerm& t = p->h_terms[base + i];
+    const int ka = regk(t.a), kb = t.b >= 0 ? regk(t.b) : -1;
+    if (ka < 0 && kb < 0) {
+      const uint32_t m = (1u << t.a) | (t.b >= 0 ? (1u << t.b) : 0u);
+      out.push_back({(int32_t)m, 0, t.w});
+    } else if (t.b < 0 || (ka >= 0 && kb >= 0)) {
+      for (int a = 0; a < (1 << R); ++a) d.pwc[a] += t.w * sgn(a, ka) * (t.b < 0 ? 1.0 : sgn(a, kb));
+    } else {
+      const int k = ka >= 0 ? ka : kb, other = ka >= 0 ? t.b : t.a;
+      hl[k].push_back({(int32_t)(1u << other), 0, t.w});
+    }
+  }
+  d.nct = (int32_t)(out.size() - off0);
+  for (int k = 0; k < kMaxR; ++k) {
+    d.nht[k] = k < R ? (int32_t)hl[k].size() : 0;
+    if (k < R) out.insert(out.end(), hl[k].begin(), hl[k].end());
+  }
+  return off0;
+}
+
 static void drop_graph(qg_plan* p) {
   if (p->gexec) hipGraphExecDestroy(p->gexec);
   p->gexec = nullptr;
@@ -636,6 +680,8 @@ static hipError_t launch_all(qg_plan* p) {
 
 static int run_launches(qg_plan* p, int ndesc) {
   qg_ctx* c = p->ctx;
+  p->h_cterms.clear();
+  std::vector<std::pair<int, size_t>> cterm_at;  // (descriptor, offset of its split terms)
   for (Launch& l : p->launches)
     if (l.kind == 0) {
       // partial rows are strided by gmax: a larger grid would overlap them
@@ -649,10 +695,26 @@ static int run_launches(qg_plan* p, int ndesc) {
       l.post = l.two && ((d.flags & (F_DRED_POST | F_PHASE_POST | F_OBS_DIAG | F_STORE_A)) != 0 ||
                          !(d.flags & F_LOAD_A) || ((d.flags & F_PHASE_PRE) && !(d.flags & F_DRED_PRE)));
       for (int b = 0; b < l.batch; ++b) desc_slots(p->h_desc[l.desc + b], l.R, l.two, l.two && !l.post);
+      if (d.flags & (F_PHASE_PRE | F_PHASE_POST | F_DRED_PRE | F_DRED_POST | F_OBS_DIAG))
+        for (int b = 0; b < l.batch; ++b)
+          cterm_at.push_back({l.desc + b, split_terms(p, p->h_desc[l.desc + b], l.R, p->h_cterms)});
       l.tab = ((d.flags & F_INIT_TAB) ? (int)sizeof(double) * (p->n + 1) : 0) +
               ((l.phase == 1 && (d.flags & (F_PHASE_PRE | F_PHASE_POST))) ? (int)sizeof(double) * 2 * (2 * d.lut_off + 1)
                                                                            : 0);
     }
+  if (!p->h_cterms.empty()) {
+    if (p->h_cterms.size() > p->cterm_cap) {
+      HIPCHK(hipStreamSynchronize(c->stream));  // the old table may still be read
+      hipFree(p->d_cterms);
+      p->d_cterms = nullptr;
+      p->cterm_cap = 0;
+      HIPCHK(hipMalloc(&p->d_cterms, sizeof(DTerm) * p->h_cterms.size()));
+      p->cterm_cap = p->h_cterms.size();
+    }
+    HIPCHK(hipMemcpyAsync(p->d_cterms, p->h_cterms.data(), sizeof(DTerm) * p->h_cterms.size(),
+                          hipMemcpyHostToDevice, c->stream));
+  }
+  for (const auto& e : cterm_at) p->h_desc[e.first].cterms = p->d_cterms + e.second;
   HIPCHK(hipMemcpyAsync(p->d_desc, p->h_desc, sizeof(PassDesc) * ndesc, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   if (p->use_graph && !p->profile) {
@@ -746,6 +808,9 @@ extern "C" int qg_plan_pass_times(const qg_plan* p, int cap, float* ms, int* kin
 // ---------------------------------------------------------------------------
 static void free_small(qg_plan* p) {
   drop_graph(p);  // its launches point into the descriptor table freed here
+  hipFree(p->d_cterms);
+  p->d_cterms = nullptr;
+  p->cterm_cap = 0;
   hipFree(p->d_map);
   hipFree(p->d_init);
   hipFree(p->d_terms);
@@ -1214,6 +1279,7 @@ static int mcclean_run(qg_plan* p, const int32_t* axes, const double* angles, co
   if (!obs.diag.empty())
     HIPCHK(hipMemcpyAsync(p->d_terms, obs.diag.data(), sizeof(DTerm) * obs.diag.size(), hipMemcpyHostToDevice,
                           c->stream));
+  p->h_terms = obs.diag;
   if (ini_state) {
     const size_t N = 1ull << n;
     double* tmp = nullptr;
@@ -1416,6 +1482,7 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
       std::copy(obs[c0 + j].diag.begin(), obs[c0 + j].diag.end(), tbuf.begin() + toff[j]);
     }
     HIPCHK(hipMemcpy(p->d_terms, tbuf.data(), sizeof(DTerm) * tbuf.size(), hipMemcpyHostToDevice));
+    p->h_terms = tbuf;
     std::vector<double> lut;
     if (phase == 1) {
       lut.resize((size_t)nb * 2 * L * lut_len * 2);

@@ -44,7 +44,8 @@ enum : uint32_t {
 };
 
 // One term of the diagonal quadratic form D(x) = sum w * z_a(x) [* z_b(x)],
-// with z_p(x) = +1 if global bit p of x is 0 else -1.  b = -1 for a one-bit term.
+// with z_p(x) = +1 if global bit p of x is 0 else -1.  b = -1 for a one-bit
+// term.  (In PassDesc::cterms, a holds a bit mask instead; see there.)
 struct DTerm {
   int32_t a, b;
   double w;
@@ -86,8 +87,16 @@ struct PassDesc {
   uint32_t zmask;
   const uint32_t* map;   // ladder map columns (n entries), or null
   const double* init_tab;// n + 1 entries when F_INIT_TAB
-  const DTerm* dterms;   // diagonal quadratic form
+  const DTerm* dterms;   // diagonal quadratic form (global bit positions a, b)
   int32_t ndterms;
+  // The same terms split by the launch's register bits (filled by the host):
+  // cterms[0, nct) have no register bit, then nht[k] terms pair register bit
+  // k with another bit; each as (a = mask of its other bits, w).  pwc[a]: the
+  // terms on register bits only, at register pattern a.
+  const DTerm* cterms;
+  int32_t nct;
+  int32_t nht[kMaxR];
+  double pwc[1 << kMaxR];
   int32_t lut_off;       // LUT index offset (F_PHASE_LUT)
   double phase_gamma;    // angle multiplying D(x) in the phase
   const double* lut;     // (re, im) pairs: exp(-i*gamma*d) at d = idx - lut_off

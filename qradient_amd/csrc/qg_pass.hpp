@@ -355,72 +355,53 @@ __device__ __forceinline__ void lds_transpose(Cx<T> (&x)[1 << R], void* lds, uin
 }
 
 // D(x) for the 2^R register values of one thread, with the thread's other
-// bits fixed: D(a) = c0 + sum_k s_k(a) h_k + sum_{k<l} s_k(a) s_l(a) pw_kl.
+// bits fixed: D(a) = c0 + sum_k s_k(a) h_k + pwc[a], s_k(a) = -1 where
+// register bit k of a is set.  The host splits the terms of D by the pass's
+// register bits (PassDesc::cterms): c0 sums the terms on other bits, h_k the
+// terms pairing register bit k with another bit, and pwc[a] (a table in the
+// descriptor) the terms on register bits only.  A term is (mask, w) with
+// value w * (-1)^popcount(x & mask) over the thread's other bits.
 template <int R>
 struct DiagParts {
   double c0;
   double h[R];
-  double pw[R][R];
 };
 
-__device__ __forceinline__ double zsgn(uint32_t x, int pos) { return ((x >> pos) & 1u) ? -1.0 : 1.0; }
-
-template <int R>
-__device__ __forceinline__ void diag_prepare(const PassDesc* __restrict__ d, uint32_t gthr, const int (&rpos)[R],
-                                             DiagParts<R>& o) {
-  o.c0 = 0.0;
-#pragma unroll
-  for (int k = 0; k < R; ++k) {
-    o.h[k] = 0.0;
-#pragma unroll
-    for (int l = 0; l < R; ++l) o.pw[k][l] = 0.0;
-  }
-  const int nt = d->ndterms;
-  for (int i = 0; i < nt; ++i) {
-    const DTerm dt = ldc(d->dterms, i);
-    const int a = dt.a, b = dt.b;
-    const double w = dt.w;
-    int ka = -1, kb = -1;
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-      if (rpos[k] == a) ka = k;
-      if (b >= 0 && rpos[k] == b) kb = k;
-    }
-    const double za = (ka >= 0) ? 1.0 : zsgn(gthr, a);
-    const double zb = (b < 0 || kb >= 0) ? 1.0 : zsgn(gthr, b);
-    if (ka < 0 && kb < 0) {
-      o.c0 += w * za * zb;
-    } else if (ka >= 0 && kb < 0) {
-#pragma unroll
-      for (int k = 0; k < R; ++k)
-        if (ka == k) o.h[k] += w * zb;
-    } else if (ka < 0 && kb >= 0) {
-#pragma unroll
-      for (int k = 0; k < R; ++k)
-        if (kb == k) o.h[k] += w * za;
-    } else {
-#pragma unroll
-      for (int k = 0; k < R; ++k)
-#pragma unroll
-        for (int l = k + 1; l < R; ++l)
-          if ((ka == k && kb == l) || (ka == l && kb == k)) o.pw[k][l] += w;
-    }
-  }
+// w * (-1)^popcount(m): the parity flips the sign bit of w
+__device__ __forceinline__ double flipw(double w, uint32_t m) {
+  const uint64_t sb = (uint64_t)(__popc(m) & 1) << 63;
+  return __longlong_as_double(__double_as_longlong(w) ^ (long long)sb);
 }
 
 template <int R>
-__device__ __forceinline__ double diag_value(const DiagParts<R>& o, int a) {
-  double v = o.c0;
+__device__ __forceinline__ void diag_prepare(const PassDesc* __restrict__ d, uint32_t gthr, DiagParts<R>& o) {
+  const DTerm* __restrict__ ct = d->cterms;
+  const int nc = ldc(&d->nct, 0);
+  double c0 = 0.0;
+  for (int i = 0; i < nc; ++i) {
+    const DTerm t = ldc(ct, i);
+    c0 += flipw(t.w, gthr & (uint32_t)t.a);
+  }
+  int off = nc;
 #pragma unroll
   for (int k = 0; k < R; ++k) {
-    const double sk = ((a >> k) & 1) ? -1.0 : 1.0;
-    v += sk * o.h[k];
-#pragma unroll
-    for (int l = k + 1; l < R; ++l) {
-      const double sl = ((a >> l) & 1) ? -1.0 : 1.0;
-      v += sk * sl * o.pw[k][l];
+    const int nh = ldc(d->nht, k);
+    double h = 0.0;
+    for (int i = 0; i < nh; ++i) {
+      const DTerm t = ldc(ct, off + i);
+      h += flipw(t.w, gthr & (uint32_t)t.a);
     }
+    o.h[k] = h;
+    off += nh;
   }
+  o.c0 = c0;
+}
+
+template <int R>
+__device__ __forceinline__ double diag_value(const PassDesc* __restrict__ d, const DiagParts<R>& o, int a) {
+  double v = o.c0 + ldc(d->pwc, a);
+#pragma unroll
+  for (int k = 0; k < R; ++k) v += ((a >> k) & 1) ? -o.h[k] : o.h[k];
   return v;
 }
 
@@ -774,11 +755,11 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
         // adjoint of a pass whose forward ends with the phase: d/dgamma at the
         // pass output, then both states are taken back through the phase
         DiagParts<R> dp;
-        diag_prepare<R>(d, gthr, rpos, dp);
+        diag_prepare<R>(d, gthr, dp);
         double xacc = 0.0;
 #pragma unroll
         for (int a = 0; a < A; ++a) {
-          const double D = diag_value<R>(dp, a);
+          const double D = diag_value<R>(d, dp, a);
           if constexpr (TWO) {
             if (RF || (flags & F_DRED_PRE)) {
               xacc += 2.0 * D * imcj(xb[a], xa[a]);
@@ -953,11 +934,11 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
     // ---- diagonal ops in the natural layout, after the gates --------------
     if (POST_DIAG && (flags & (F_DRED_POST | F_PHASE_POST | F_OBS_DIAG))) {
       DiagParts<R> dp;
-      diag_prepare<R>(d, gst, rpos, dp);
+      diag_prepare<R>(d, gst, dp);
       double xacc = 0.0;
 #pragma unroll
       for (int a = 0; a < A; ++a) {
-        const double D = diag_value<R>(dp, a);
+        const double D = diag_value<R>(d, dp, a);
         if constexpr (TWO) {
           if (flags & F_DRED_POST) xacc += 2.0 * D * imcj(xb[a], xa[a]);
         }

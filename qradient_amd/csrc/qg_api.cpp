@@ -133,11 +133,11 @@ extern "C" int qg_ctx_create(int device, void* stream, qg_ctx** out) {
 
 extern "C" int qg_ctx_destroy(qg_ctx* c) {
   if (!c) return QG_OK;
-  hipSetDevice(c->device);
-  if (c->stream) hipStreamSynchronize(c->stream);
-  if (c->ev0) hipEventDestroy(c->ev0);
-  if (c->ev1) hipEventDestroy(c->ev1);
-  if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return QG_OK;
 }
@@ -466,12 +466,12 @@ static int alloc_small(qg_plan* p, int ndesc, int nrows, int nout) {
 
 extern "C" int qg_plan_destroy(qg_plan* p) {
   if (!p) return QG_OK;
-  hipSetDevice(p->ctx->device);
-  hipStreamSynchronize(p->ctx->stream);
-  if (p->ws_owned && p->ws) hipFree(p->ws);
+  (void)hipSetDevice(p->ctx->device);
+  (void)hipStreamSynchronize(p->ctx->stream);
+  if (p->ws_owned && p->ws) (void)hipFree(p->ws);
   free_small(p);
   drop_graph(p);
-  for (hipEvent_t e : p->pev) hipEventDestroy(e);
+  for (hipEvent_t e : p->pev) (void)hipEventDestroy(e);
   delete p;
   return QG_OK;
 }
@@ -487,8 +487,8 @@ extern "C" int qg_plan_set_workspace(qg_plan* p, void* dev, size_t bytes) {
   if (bytes < p->ws_bytes) return fail(QG_ERR_INVALID, "workspace smaller than qg_plan_workspace_size");
   if (((uintptr_t)dev) % 256 != 0) return fail(QG_ERR_INVALID, "workspace must be 256-byte aligned");
   if (p->ws_owned && p->ws) {
-    hipStreamSynchronize(p->ctx->stream);
-    hipFree(p->ws);
+    (void)hipStreamSynchronize(p->ctx->stream);
+    (void)hipFree(p->ws);
   }
   p->ws = (char*)dev;
   p->ws_owned = false;
@@ -511,7 +511,7 @@ extern "C" int qg_plan_get_state(qg_plan* p, double* out) {
   hipError_t e = simple_convert_out(p->dtype, p->last_state_ptr, tmp, p->n, p->ctx->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(out, tmp, N * 16, hipMemcpyDeviceToHost, p->ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(p->ctx->stream);
-  hipFree(tmp);
+  (void)hipFree(tmp);
   if (e != hipSuccess) return fail(QG_ERR_HIP, std::string("qg_plan_get_state: ") + hipGetErrorString(e));
   if (p->state_sign)
     for (size_t k = 0; k < 2 * N; ++k) out[k] = -out[k];
@@ -662,7 +662,7 @@ static size_t split_terms(const qg_plan* p, PassDesc& d, int R, std::vector<DTer
 }
 
 static void drop_graph(qg_plan* p) {
-  if (p->gexec) hipGraphExecDestroy(p->gexec);
+  if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
   p->gexec = nullptr;
   p->gsig.clear();
 }
@@ -705,7 +705,7 @@ static int run_launches(qg_plan* p, int ndesc) {
   if (!p->h_cterms.empty()) {
     if (p->h_cterms.size() > p->cterm_cap) {
       HIPCHK(hipStreamSynchronize(c->stream));  // the old table may still be read
-      hipFree(p->d_cterms);
+      (void)hipFree(p->d_cterms);
       p->d_cterms = nullptr;
       p->cterm_cap = 0;
       HIPCHK(hipMalloc(&p->d_cterms, sizeof(DTerm) * p->h_cterms.size()));
@@ -737,7 +737,7 @@ static int run_launches(qg_plan* p, int ndesc) {
         if (e == hipSuccess) e = e2;
       }
       if (e == hipSuccess) e = hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0);
-      if (g) hipGraphDestroy(g);
+      if (g) (void)hipGraphDestroy(g);
       if (e == hipSuccess) {
         p->gsig.swap(sig);
       } else {
@@ -808,25 +808,25 @@ extern "C" int qg_plan_pass_times(const qg_plan* p, int cap, float* ms, int* kin
 // ---------------------------------------------------------------------------
 static void free_small(qg_plan* p) {
   drop_graph(p);  // its launches point into the descriptor table freed here
-  hipFree(p->d_cterms);
+  (void)hipFree(p->d_cterms);
   p->d_cterms = nullptr;
   p->cterm_cap = 0;
-  hipFree(p->d_map);
-  hipFree(p->d_init);
-  hipFree(p->d_terms);
-  hipFree(p->d_xy);
-  hipFree(p->d_lut);
-  hipFree(p->d_part);
-  hipFree(p->d_fin);
-  hipFree(p->d_scale);
-  hipFree(p->d_out);
-  hipFree(p->d_dot);
-  hipFree(p->d_desc);
-  if (p->h_desc) hipHostFree(p->h_desc);
-  if (p->h_out) hipHostFree(p->h_out);
-  if (p->h_arena) hipHostFree(p->h_arena);
-  hipFree(p->d_arena);
-  hipFree(p->d_fstate);
+  (void)hipFree(p->d_map);
+  (void)hipFree(p->d_init);
+  (void)hipFree(p->d_terms);
+  (void)hipFree(p->d_xy);
+  (void)hipFree(p->d_lut);
+  (void)hipFree(p->d_part);
+  (void)hipFree(p->d_fin);
+  (void)hipFree(p->d_scale);
+  (void)hipFree(p->d_out);
+  (void)hipFree(p->d_dot);
+  (void)hipFree(p->d_desc);
+  if (p->h_desc) (void)hipHostFree(p->h_desc);
+  if (p->h_out) (void)hipHostFree(p->h_out);
+  if (p->h_arena) (void)hipHostFree(p->h_arena);
+  (void)hipFree(p->d_arena);
+  (void)hipFree(p->d_fstate);
   p->h_arena = nullptr;
   p->d_arena = nullptr;
   p->d_fstate = nullptr;
@@ -855,7 +855,7 @@ static int mcclean_layout(qg_plan* p) {
     // (1024-thread variants exist for those); otherwise fall back to 13
     const size_t S = (size_t)(1ull << n) * 8;
     size_t fb = 0, tb = 0;
-    hipMemGetInfo(&fb, &tb);
+    HIPCHK(hipMemGetInfo(&fb, &tb));
     if (p->ws_owned && p->ws) fb += p->ws_bytes;
     const bool hist_fits = p->ckpt_req <= 0 && (size_t)(P * L + 4) * S <= (size_t)(0.85 * (double)fb);
     const bool fp64_tiles = p->lds64_req == 1 || (p->lds64_req < 0 && P * L > 200);
@@ -872,7 +872,7 @@ static int mcclean_layout(qg_plan* p) {
   // once per pass, at the store)
   p->lds64 = p->dtype == QG_C64 && n >= 4 && (p->lds64_req == 1 || (p->lds64_req < 0 && P * L > 200));
   size_t free_b = 0, total_b = 0;
-  hipMemGetInfo(&free_b, &total_b);
+  HIPCHK(hipMemGetInfo(&free_b, &total_b));
   if (p->ws_owned && p->ws) free_b += p->ws_bytes;
   const size_t budget = (size_t)(0.85 * (double)free_b);
   const size_t hist_bytes = (size_t)(P * L + 4) * p->S;
@@ -1127,7 +1127,7 @@ static int upload_finalize(qg_plan* p, const std::vector<std::vector<int>>& out_
   for (int r = 0; r < p->nrows; ++r) buf.push_back(r < (int)cnt.size() ? cnt[r] : 0);
   if (buf.size() > p->fin_cap) {
     HIPCHK(hipStreamSynchronize(p->ctx->stream));
-    hipFree(p->d_fin);
+    (void)hipFree(p->d_fin);
     p->d_fin = nullptr;
     HIPCHK(hipMalloc(&p->d_fin, sizeof(int) * buf.size()));
     p->fin_cap = buf.size();
@@ -1162,8 +1162,8 @@ struct Arena {
 static int ensure_arena(qg_plan* p, size_t bytes) {
   if (bytes <= p->arena_cap) return QG_OK;
   HIPCHK(hipStreamSynchronize(p->ctx->stream));
-  if (p->h_arena) hipHostFree(p->h_arena);
-  hipFree(p->d_arena);
+  if (p->h_arena) (void)hipHostFree(p->h_arena);
+  (void)hipFree(p->d_arena);
   p->h_arena = nullptr;
   p->d_arena = nullptr;
   p->arena_cap = 0;
@@ -1287,7 +1287,7 @@ static int mcclean_run(qg_plan* p, const int32_t* axes, const double* angles, co
     hipError_t e = hipMemcpyAsync(tmp, ini_state, N * 16, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = simple_convert_in(p->dtype, tmp, wsp(p, p->off_ini), n, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    hipFree(tmp);
+    (void)hipFree(tmp);
     if (e != hipSuccess) return fail(QG_ERR_HIP, std::string("ini_state upload: ") + hipGetErrorString(e));
   }
   int nd = 0;
@@ -1367,7 +1367,10 @@ extern "C" int qg_qaoa_plan(qg_ctx* ctx, int n, int p_layers, int dtype, int bat
   p->S = (size_t)(1ull << n) * (dtype == QG_C64 ? 8 : 16);
   const size_t per_circuit = (size_t)(P * p_layers + 2) * p->S;
   size_t free_b = 0, total_b = 0;
-  hipMemGetInfo(&free_b, &total_b);
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+    delete p;
+    return fail(QG_ERR_HIP, "hipMemGetInfo failed");
+  }
   const size_t budget = (size_t)(0.80 * (double)free_b);
   if (per_circuit > budget) {
     delete p;
@@ -1385,7 +1388,7 @@ extern "C" int qg_qaoa_plan(qg_ctx* ctx, int n, int p_layers, int dtype, int bat
     qg_plan_destroy(p);
     return r;
   }
-  hipFree(p->d_terms);
+  (void)hipFree(p->d_terms);
   p->d_terms = nullptr;
   p->max_terms = n + n * (n - 1) / 2;
   hipError_t e = hipMalloc(&p->d_terms, sizeof(DTerm) * (size_t)p->max_terms * p->chunk);
@@ -1463,7 +1466,7 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
   if (phase == 1) {
     const int need = chunk * 2 * L * lut_len * 2;
     if (need > p->lut_len) {
-      hipFree(p->d_lut);
+      (void)hipFree(p->d_lut);
       p->d_lut = nullptr;
       HIPCHK(hipMalloc(&p->d_lut, sizeof(double) * need));
       p->lut_len = need;
@@ -1706,15 +1709,15 @@ static int state_init(qg_state* s, char ini) {
 
 extern "C" int qg_state_destroy(qg_state* s) {
   if (!s) return QG_OK;
-  hipSetDevice(s->ctx->device);
-  hipStreamSynchronize(s->ctx->stream);
-  hipFree(s->v);
-  hipFree(s->tmp);
-  hipFree(s->d_map);
-  hipFree(s->d_terms);
-  hipFree(s->d_xy);
-  hipFree(s->d_tab);
-  hipFree(s->d_dot);
+  (void)hipSetDevice(s->ctx->device);
+  (void)hipStreamSynchronize(s->ctx->stream);
+  (void)hipFree(s->v);
+  (void)hipFree(s->tmp);
+  (void)hipFree(s->d_map);
+  (void)hipFree(s->d_terms);
+  (void)hipFree(s->d_xy);
+  (void)hipFree(s->d_tab);
+  (void)hipFree(s->d_dot);
   delete s;
   return QG_OK;
 }
@@ -1766,7 +1769,7 @@ extern "C" int qg_state_set(qg_state* s, const double* vec) {
   hipError_t e = hipMemcpyAsync(tmp, vec, N * 16, hipMemcpyHostToDevice, s->ctx->stream);
   if (e == hipSuccess) e = simple_convert_in(s->dtype, tmp, s->v, s->n, s->ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(s->ctx->stream);
-  hipFree(tmp);
+  (void)hipFree(tmp);
   if (e != hipSuccess) return fail(QG_ERR_HIP, std::string("qg_state_set: ") + hipGetErrorString(e));
   return QG_OK;
 }
@@ -1780,7 +1783,7 @@ extern "C" int qg_state_get(qg_state* s, double* vec) {
   hipError_t e = simple_convert_out(s->dtype, s->v, tmp, s->n, s->ctx->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(vec, tmp, N * 16, hipMemcpyDeviceToHost, s->ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(s->ctx->stream);
-  hipFree(tmp);
+  (void)hipFree(tmp);
   if (e != hipSuccess) return fail(QG_ERR_HIP, std::string("qg_state_get: ") + hipGetErrorString(e));
   return QG_OK;
 }
@@ -1922,7 +1925,7 @@ static int relayout(qg_plan* p) {
   CHK(mcclean_layout(p));
   if (p->ws && p->ws_bytes > old_ws) {
     if (!p->ws_owned) return fail(QG_ERR_INVALID, "the new layout needs a larger workspace than the one supplied");
-    hipFree(p->ws);
+    (void)hipFree(p->ws);
     p->ws = nullptr;
     p->ws_owned = false;
   }

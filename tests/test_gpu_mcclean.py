@@ -224,3 +224,14 @@ def test_wide_tiles_c64_matches_oracle(gpu, n, L, entangler):
     psi = c.final_state()
     ref = sv.mcclean_forward(n, axes, angles, entangler=entangler)
     assert np.max(np.abs(psi - ref)) <= 1e-5
+
+
+@pytest.mark.parametrize('n,L', [(17, 3), (22, 2)])
+@pytest.mark.parametrize('dtype,tol', [('complex128', 1e-12), ('complex64', 1e-6)])
+def test_z_terms_on_pass_path(gpu, n, L, dtype, tol):
+    """One-bit terms with the observable on the last (high-bit) pass."""
+    axes, angles, obs = _case(n, L, 120 + n, 'zmix')
+    E, g = _run(n, L, axes, angles, obs, dtype=dtype)
+    Eo, go = sv.mcclean_grad_run_adjoint(n, axes, angles, obs)
+    assert abs(E - Eo) <= tol * max(1.0, abs(Eo))
+    assert rel_err(g, go) <= tol

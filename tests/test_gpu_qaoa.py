@@ -79,3 +79,27 @@ def test_batched_equals_single_runs(gpu, dtype, tol):
     # each circuit alone gives bitwise the same result as inside the batch
     E1, g1 = QaoaBatch(n, p, 1, dtype=dtype).grad_run(obs[3:4], betas[3:4], gammas[3:4])
     assert E1[0] == E[3] and np.array_equal(g1[0], grad[3])
+
+
+@pytest.mark.parametrize('n,p,integral', [(15, 2, False), (18, 3, False), (16, 2, True)])
+@pytest.mark.parametrize('dtype,tol', [('complex128', 1e-12), ('complex64', 1e-6)])
+def test_qaoa_pass_path_diag_terms(gpu, n, p, integral, dtype, tol):
+    """Tile-pass QAOA with one- and two-bit terms (sincos phase for real
+    weights, the phase table for integer weights); the terms are split per
+    launch by its register bits."""
+    from qradient_amd.circuit_logic import Qaoa
+    rs = np.random.RandomState(40 + n)
+    zz = np.full((n, n), None)
+    for i in range(n):
+        j = (i + 1 + i % 4) % n
+        a, c = min(i, j), max(i, j)
+        if a != c:
+            zz[a, c] = float(rs.randint(-2, 3) or 1) if integral else float(rs.standard_normal())
+    z = np.array([(float(rs.randint(-1, 2) or 2) if integral else 0.5 * rs.standard_normal()) if q % 3 == 1 else None
+                  for q in range(n)], dtype=object)
+    obs = {'zz': zz, 'z': z}
+    b, g = np.pi * rs.rand(p), np.pi * rs.rand(p)
+    E, grad = Qaoa(n, obs, p, dtype=dtype).grad_run(b, g)
+    Eo, go = sv.qaoa_grad_run(n, obs, b, g)
+    assert abs(E - Eo) <= tol * max(1.0, abs(Eo))
+    assert rel_err(grad, go) <= tol, rel_err(grad, go)

@@ -98,6 +98,18 @@ struct Stor<float> {
   }
   // byte-offset forms
   template <class T>
+  static __device__ __forceinline__ Cx<T> ld_o(const void* p, uint32_t off) {
+    const f2v v = *(const gf2v*)((const char*)p + off);
+    return {(T)v.x, (T)v.y};
+  }
+  template <class T>
+  static __device__ __forceinline__ void st_o(void* p, uint32_t off, Cx<T> v) {
+    f2v w;
+    w.x = (float)v.re;
+    w.y = (float)v.im;
+    *(gf2v*)((char*)p + off) = w;
+  }
+  template <class T>
   static __device__ __forceinline__ void lds_st_b(void* lds, uint32_t off, Cx<T> v) {
     f2v w;
     w.x = (float)v.re;
@@ -148,6 +160,18 @@ struct Stor<double> {
   static __device__ __forceinline__ Cx<T> lds_ld(const void* lds, uint32_t i) {
     const d2v v = ((const ld2v*)lds)[i];
     return {(T)v.x, (T)v.y};
+  }
+  template <class T>
+  static __device__ __forceinline__ Cx<T> ld_o(const void* p, uint32_t off) {
+    const d2v v = *(const gd2v*)((const char*)p + off);
+    return {(T)v.x, (T)v.y};
+  }
+  template <class T>
+  static __device__ __forceinline__ void st_o(void* p, uint32_t off, Cx<T> v) {
+    d2v w;
+    w.x = (double)v.re;
+    w.y = (double)v.im;
+    *(gd2v*)((char*)p + off) = w;
   }
   template <class T>
   static __device__ __forceinline__ void lds_st_b(void* lds, uint32_t off, Cx<T> v) {
@@ -335,22 +359,32 @@ __device__ __forceinline__ double red_bit(const Cd (&l)[1 << R], const Cx<T> (&p
   return s0 + s1;
 }
 
+// The uniform term of an LDS byte offset, in a scalar register: computed by
+// the scalar unit, and kept from being folded back into per-lane swizzles.
+template <class S>
+__device__ __forceinline__ uint32_t swz_u(uint32_t j) {
+  constexpr int SH = sizeof(S) == 4 ? 3 : 4;
+  return __builtin_amdgcn_readfirstlane(swz<S>(j) << SH);
+}
+
 // The swizzle is GF(2)-linear and the thread and register parts of a local
 // index have disjoint bits, so slot(lidx) = swz(thread part) ^ swz(a << lo):
 // one per-thread base per layout and a uniform term per register.
 // Byte offsets are formed directly (the slot shift is linear too), so each
-// access costs one VALU XOR with a uniform term.
+// access costs one VALU XOR with a scalar term.
 template <class S, int R, class T>
 __device__ __forceinline__ void lds_transpose(Cx<T> (&x)[1 << R], void* lds, uint32_t tid, int lo_from, int lo_to) {
   constexpr int A = 1 << R;
   constexpr int SH = sizeof(S) == 4 ? 3 : 4;
+  lo_from = __builtin_amdgcn_readfirstlane(lo_from);
+  lo_to = __builtin_amdgcn_readfirstlane(lo_to);
   const uint32_t bw = swz<S>(lidx<R>(tid, 0, lo_from)) << SH;
   const uint32_t br = swz<S>(lidx<R>(tid, 0, lo_to)) << SH;
 #pragma unroll
-  for (int a = 0; a < A; ++a) Stor<S>::lds_st_b(lds, bw ^ (swz<S>((uint32_t)a << lo_from) << SH), x[a]);
+  for (int a = 0; a < A; ++a) Stor<S>::lds_st_b(lds, bw ^ swz_u<S>((uint32_t)a << lo_from), x[a]);
   __syncthreads();
 #pragma unroll
-  for (int a = 0; a < A; ++a) x[a] = Stor<S>::template lds_ld_b<T>(lds, br ^ (swz<S>((uint32_t)a << lo_to) << SH));
+  for (int a = 0; a < A; ++a) x[a] = Stor<S>::template lds_ld_b<T>(lds, br ^ swz_u<S>((uint32_t)a << lo_to));
   __syncthreads();
 }
 
@@ -528,28 +562,32 @@ __device__ __forceinline__ void gate_bit_s(Cx<T> (&x)[1 << R], int type, double 
 
 // Global addresses (amplitude indices) of a thread's 2^R amplitudes in a tile
 // whose thread-0 register-0 index is g: plain, or through the ladder map.
-template <int R>
+// SH > 0: byte offsets (index << SH) instead of indices.  The register part
+// of an address is uniform: it is formed by the scalar unit, so each address
+// costs one VALU OR / XOR with the thread's base.
+template <int R, int SH = 0>
 __device__ __forceinline__ void tile_addresses(const PassDesc* __restrict__ d, bool gather, uint32_t g,
                                                const int (&rpos)[R], const uint32_t (&rbit)[R], int n,
                                                uint32_t (&adr)[1 << R]) {
   if (gather) {
-    const uint32_t mbase = apply_map(d->map, g, n);
+    const uint32_t mbase = apply_map(d->map, g, n) << SH;
 #pragma unroll
     for (int a = 0; a < (1 << R); ++a) {
-      uint32_t v = mbase;
+      uint32_t u = 0;
 #pragma unroll
       for (int k = 0; k < R; ++k)
-        if (a & (1 << k)) v ^= ldc(d->map, rpos[k]);
-      adr[a] = v;
+        if (a & (1 << k)) u ^= ldc(d->map, rpos[k]);
+      adr[a] = mbase ^ __builtin_amdgcn_readfirstlane(u << SH);
     }
   } else {
+    const uint32_t gb = g << SH;
 #pragma unroll
     for (int a = 0; a < (1 << R); ++a) {
-      uint32_t v = g;
+      uint32_t u = 0;
 #pragma unroll
       for (int k = 0; k < R; ++k)
-        if (a & (1 << k)) v |= rbit[k];
-      adr[a] = v;
+        if (a & (1 << k)) u |= rbit[k];
+      adr[a] = gb | __builtin_amdgcn_readfirstlane(u << SH);
     }
   }
 }
@@ -590,6 +628,7 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
   constexpr int A = 1 << R;
   typedef typename Store<S>::G G;  // HBM element type
   typedef typename Store<S>::L LT; // LDS tile element type
+  constexpr int GSH = sizeof(G) == 4 ? 3 : 4;  // log2 bytes per stored amplitude
   // WIDE: a 1024-thread workgroup (14 local bits), which must fit 128 VGPRs
   // and leaves no LDS for per-thread accumulators: no prefetch in adjoint
   // passes, per-tile wave sums
@@ -689,13 +728,13 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
     uint32_t tid = tid_in;
     asm volatile("" : "+v"(tid));
     uint32_t adr[A];
-    tile_addresses<R>(d, gather, gl_tile(tt, c, p, K) | gl_local(tid, c, p), rpos, rbit, n, adr);
+    tile_addresses<R, BIG ? 0 : GSH>(d, gather, gl_tile(tt, c, p, K) | gl_local(tid, c, p), rpos, rbit, n, adr);
     // every path defines every register (zeros when a state is not loaded),
     // so no value is carried from one tile to the next
     if (load_a) {
 #pragma unroll
       for (int a = 0; a < A; ++a)
-        ra[a] = BIG ? Stor<G>::template ld_big<G>(d->a_in, adr[a]) : Stor<G>::template ld<G>(d->a_in, adr[a]);
+        ra[a] = BIG ? Stor<G>::template ld_big<G>(d->a_in, adr[a]) : Stor<G>::template ld_o<G>(d->a_in, adr[a]);
     } else {
 #pragma unroll
       for (int a = 0; a < A; ++a) ra[a] = {(G)0, (G)0};
@@ -704,7 +743,7 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
       if (load_b) {
 #pragma unroll
         for (int a = 0; a < A; ++a)
-          rb[a] = BIG ? Stor<G>::template ld_big<G>(d->b_in, adr[a]) : Stor<G>::template ld<G>(d->b_in, adr[a]);
+          rb[a] = BIG ? Stor<G>::template ld_big<G>(d->b_in, adr[a]) : Stor<G>::template ld_o<G>(d->b_in, adr[a]);
       } else {
 #pragma unroll
         for (int a = 0; a < A; ++a) rb[a] = {(G)0, (G)0};
@@ -805,9 +844,10 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
         }
         const double msum = m0 + m1;
         constexpr int SH = sizeof(LT) == 4 ? 3 : 4;
+        const int tu = __builtin_amdgcn_readfirstlane(t);
+        const uint32_t base = swz<LT>(tid) << SH;  // tid < 2^t: thread and register bits are disjoint
 #pragma unroll
-        for (int a = 0; a < A; ++a)
-          Stor<LT>::lds_st_b(lds, swz<LT>(tid | ((uint32_t)a << t)) << SH, xa[a]);
+        for (int a = 0; a < A; ++a) Stor<LT>::lds_st_b(lds, base ^ swz_u<LT>((uint32_t)a << tu), xa[a]);
         __syncthreads();
         auto accum = [&](int lb, double v) __attribute__((always_inline)) {
           if constexpr (ACCL) {
@@ -827,10 +867,9 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
         // current bit is reduced
         typedef Cx<LT> Raw[A];
         auto rd = [&](int e, Raw& P) __attribute__((always_inline)) {
-          const uint32_t pb = swz<LT>(tid ^ (1u << (e & 255))) << SH;
+          const uint32_t pb = base ^ swz_u<LT>(1u << (e & 255));  // the partner thread tid ^ 2^lb
 #pragma unroll
-          for (int a = 0; a < A; ++a)
-            P[a] = Stor<LT>::template lds_ld_b<LT>(lds, pb ^ (swz<LT>((uint32_t)a << t) << SH));
+          for (int a = 0; a < A; ++a) P[a] = Stor<LT>::template lds_ld_b<LT>(lds, pb ^ swz_u<LT>((uint32_t)a << tu));
         };
         auto use = [&](int e, const Raw& P) __attribute__((always_inline)) {
           const int lb = e & 255;
@@ -979,12 +1018,12 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
 
     // ---- store: addresses recomputed from the laundered base ----------------
     uint32_t adr[A];
-    tile_addresses<R>(d, (flags & F_SCATTER) != 0, gst, rpos, rbit, n, adr);
+    tile_addresses<R, BIG ? 0 : GSH>(d, (flags & F_SCATTER) != 0, gst, rpos, rbit, n, adr);
     if (!RF && (flags & F_STORE_A)) {
 #pragma unroll
       for (int a = 0; a < A; ++a) {
         if (BIG) Stor<G>::st_big(d->a_out, adr[a], xa[a]);
-        else Stor<G>::st(d->a_out, adr[a], xa[a]);
+        else Stor<G>::st_o(d->a_out, adr[a], xa[a]);
       }
     }
     if constexpr (TWO) {
@@ -992,7 +1031,7 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
 #pragma unroll
         for (int a = 0; a < A; ++a) {
           if (BIG) Stor<G>::st_big(d->b_out, adr[a], xb[a]);
-          else Stor<G>::st(d->b_out, adr[a], xb[a]);
+          else Stor<G>::st_o(d->b_out, adr[a], xb[a]);
         }
       }
     }

@@ -109,6 +109,19 @@ struct Stor<float> {
     w.y = (float)v.im;
     *(gf2v*)((char*)p + off) = w;
   }
+  // by absolute LDS address
+  template <class T>
+  static __device__ __forceinline__ void lds_st_a(uint32_t addr, Cx<T> v) {
+    f2v w;
+    w.x = (float)v.re;
+    w.y = (float)v.im;
+    *(lf2v*)(uintptr_t)addr = w;
+  }
+  template <class T>
+  static __device__ __forceinline__ Cx<T> lds_ld_a(uint32_t addr) {
+    const f2v v = *(const lf2v*)(uintptr_t)addr;
+    return {(T)v.x, (T)v.y};
+  }
   template <class T>
   static __device__ __forceinline__ void lds_st_b(void* lds, uint32_t off, Cx<T> v) {
     f2v w;
@@ -172,6 +185,19 @@ struct Stor<double> {
     w.x = (double)v.re;
     w.y = (double)v.im;
     *(gd2v*)((char*)p + off) = w;
+  }
+  // by absolute LDS address
+  template <class T>
+  static __device__ __forceinline__ void lds_st_a(uint32_t addr, Cx<T> v) {
+    d2v w;
+    w.x = (double)v.re;
+    w.y = (double)v.im;
+    *(ld2v*)(uintptr_t)addr = w;
+  }
+  template <class T>
+  static __device__ __forceinline__ Cx<T> lds_ld_a(uint32_t addr) {
+    const d2v v = *(const ld2v*)(uintptr_t)addr;
+    return {(T)v.x, (T)v.y};
   }
   template <class T>
   static __device__ __forceinline__ void lds_st_b(void* lds, uint32_t off, Cx<T> v) {
@@ -359,6 +385,14 @@ __device__ __forceinline__ double red_bit(const Cd (&l)[1 << R], const Cx<T> (&p
   return s0 + s1;
 }
 
+// LDS address of a pointer into the kernel's dynamic LDS.  k_pass has no
+// static LDS, so its dynamic LDS starts at address 0: a per-thread base that
+// includes it can be XORed with offsets (XOR equals addition on bits the base
+// leaves clear), and the accesses need no separate base add.
+__device__ __forceinline__ uint32_t lds_addr(const void* p) {
+  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) char*)p;
+}
+
 // The uniform term of an LDS byte offset, in a scalar register: computed by
 // the scalar unit, and kept from being folded back into per-lane swizzles.
 template <class S>
@@ -378,13 +412,16 @@ __device__ __forceinline__ void lds_transpose(Cx<T> (&x)[1 << R], void* lds, uin
   constexpr int SH = sizeof(S) == 4 ? 3 : 4;
   lo_from = __builtin_amdgcn_readfirstlane(lo_from);
   lo_to = __builtin_amdgcn_readfirstlane(lo_to);
-  const uint32_t bw = swz<S>(lidx<R>(tid, 0, lo_from)) << SH;
-  const uint32_t br = swz<S>(lidx<R>(tid, 0, lo_to)) << SH;
+  uint32_t bw = lds_addr(lds) + (swz<S>(lidx<R>(tid, 0, lo_from)) << SH);
+  uint32_t br = lds_addr(lds) + (swz<S>(lidx<R>(tid, 0, lo_to)) << SH);
+  // opaque bases: otherwise the shift common to base and scalar term is
+  // factored out of the XOR, costing a second VALU op per address
+  asm volatile("" : "+v"(bw), "+v"(br));
 #pragma unroll
-  for (int a = 0; a < A; ++a) Stor<S>::lds_st_b(lds, bw ^ swz_u<S>((uint32_t)a << lo_from), x[a]);
+  for (int a = 0; a < A; ++a) Stor<S>::lds_st_a(bw ^ swz_u<S>((uint32_t)a << lo_from), x[a]);
   __syncthreads();
 #pragma unroll
-  for (int a = 0; a < A; ++a) x[a] = Stor<S>::template lds_ld_b<T>(lds, br ^ swz_u<S>((uint32_t)a << lo_to));
+  for (int a = 0; a < A; ++a) x[a] = Stor<S>::template lds_ld_a<T>(br ^ swz_u<S>((uint32_t)a << lo_to));
   __syncthreads();
 }
 
@@ -570,7 +607,8 @@ __device__ __forceinline__ void tile_addresses(const PassDesc* __restrict__ d, b
                                                const int (&rpos)[R], const uint32_t (&rbit)[R], int n,
                                                uint32_t (&adr)[1 << R]) {
   if (gather) {
-    const uint32_t mbase = apply_map(d->map, g, n) << SH;
+    uint32_t mbase = apply_map(d->map, g, n) << SH;
+    asm volatile("" : "+v"(mbase));  // opaque (see lds_transpose)
 #pragma unroll
     for (int a = 0; a < (1 << R); ++a) {
       uint32_t u = 0;
@@ -580,7 +618,8 @@ __device__ __forceinline__ void tile_addresses(const PassDesc* __restrict__ d, b
       adr[a] = mbase ^ __builtin_amdgcn_readfirstlane(u << SH);
     }
   } else {
-    const uint32_t gb = g << SH;
+    uint32_t gb = g << SH;
+    asm volatile("" : "+v"(gb));  // opaque (see lds_transpose)
 #pragma unroll
     for (int a = 0; a < (1 << R); ++a) {
       uint32_t u = 0;
@@ -845,9 +884,11 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
         const double msum = m0 + m1;
         constexpr int SH = sizeof(LT) == 4 ? 3 : 4;
         const int tu = __builtin_amdgcn_readfirstlane(t);
-        const uint32_t base = swz<LT>(tid) << SH;  // tid < 2^t: thread and register bits are disjoint
+        // tid < 2^t: thread and register bits are disjoint
+        uint32_t base = lds_addr(lds) + (swz<LT>(tid) << SH);
+        asm volatile("" : "+v"(base));  // opaque (see lds_transpose)
 #pragma unroll
-        for (int a = 0; a < A; ++a) Stor<LT>::lds_st_b(lds, base ^ swz_u<LT>((uint32_t)a << tu), xa[a]);
+        for (int a = 0; a < A; ++a) Stor<LT>::lds_st_a(base ^ swz_u<LT>((uint32_t)a << tu), xa[a]);
         __syncthreads();
         auto accum = [&](int lb, double v) __attribute__((always_inline)) {
           if constexpr (ACCL) {
@@ -869,7 +910,7 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
         auto rd = [&](int e, Raw& P) __attribute__((always_inline)) {
           const uint32_t pb = base ^ swz_u<LT>(1u << (e & 255));  // the partner thread tid ^ 2^lb
 #pragma unroll
-          for (int a = 0; a < A; ++a) P[a] = Stor<LT>::template lds_ld_b<LT>(lds, pb ^ swz_u<LT>((uint32_t)a << tu));
+          for (int a = 0; a < A; ++a) P[a] = Stor<LT>::template lds_ld_a<LT>(pb ^ swz_u<LT>((uint32_t)a << tu));
         };
         auto use = [&](int e, const Raw& P) __attribute__((always_inline)) {
           const int lb = e & 255;

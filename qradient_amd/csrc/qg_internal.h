@@ -81,8 +81,9 @@ struct PassDesc {
   int32_t slb[kMaxSlots];  // local bit of the slot
   int32_t nstage;          // stages that hold a gate or a reduction (later stages are skipped)
   // reduce-first adjoint passes: the owned thread bits (local bits below t),
-  // X / Y ones as a list (bit | type << 8) and Z ones as a mask
+  // X / Y ones as a list of bits (the nx X bits first) and Z ones as a mask
   int32_t nxy;
+  int32_t nx;
   int32_t xy[kMaxK];
   uint32_t zmask;
   const uint32_t* map;   // ladder map columns (n entries), or null

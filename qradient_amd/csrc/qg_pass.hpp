@@ -905,25 +905,25 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
         }
         // X / Y: the partner amplitudes of A (thread tid ^ 2^lb) from LDS,
         // raw; with FP32 tiles the next bit's reads are in flight while the
-        // current bit is reduced
+        // current bit is reduced.  The X bits and the Y bits are separate
+        // lists, so each loop has one fixed formula.
         typedef Cx<LT> Raw[A];
-        auto rd = [&](int e, Raw& P) __attribute__((always_inline)) {
-          const uint32_t pb = base ^ swz_u<LT>(1u << (e & 255));  // the partner thread tid ^ 2^lb
+        auto rd = [&](int lb, Raw& P) __attribute__((always_inline)) {
+          const uint32_t pb = base ^ swz_u<LT>(1u << lb);  // the partner thread tid ^ 2^lb
 #pragma unroll
           for (int a = 0; a < A; ++a) P[a] = Stor<LT>::template lds_ld_a<LT>(pb ^ swz_u<LT>((uint32_t)a << tu));
         };
-        auto use = [&](int e, const Raw& P) __attribute__((always_inline)) {
-          const int lb = e & 255;
-          const bool isx = (e >> 8) == G_X;
+        auto use = [&](auto xc, int lb, const Raw& P) __attribute__((always_inline)) {
+          constexpr bool isx = decltype(xc)::value;
           double v0 = 0.0, v1 = 0.0;
 #pragma unroll
           for (int a = 0; a < A; ++a) {
             const double pr = (double)P[a].re, pi = (double)P[a].im;
             double& acc = (a & 1) ? v1 : v0;
-            if (isx) {  // Im(conj(B_x) A_{x^q})
+            if constexpr (isx) {  // Im(conj(B_x) A_{x^q})
               acc = fmd(xb[a].re, pi, acc);
               acc = fmd(-xb[a].im, pr, acc);
-            } else {    // Re(conj(B_x) A_{x^q}), signed below
+            } else {              // Re(conj(B_x) A_{x^q}), signed below
               acc = fmd(xb[a].re, pr, acc);
               acc = fmd(xb[a].im, pi, acc);
             }
@@ -932,27 +932,31 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
           // Y: -sg * Re(...), sg = +1 where the bit is 0
           accum(lb, isx ? v : (((tid >> lb) & 1u) ? v : -v));
         };
-        const int nxy = ldc(&d->nxy, 0);
-        if constexpr (sizeof(LT) == 4 && !WIDE) {
-          Raw P0, P1;
-          if (nxy > 0) rd(ldc(d->xy, 0), P0);
-          int i = 0;
+        auto run = [&](auto xc, int i0, int i1) __attribute__((always_inline)) {
+          if constexpr (sizeof(LT) == 4 && !WIDE) {
+            Raw P0, P1;
+            if (i0 < i1) rd(ldc(d->xy, i0), P0);
+            int i = i0;
 #pragma unroll 1
-          for (; i + 1 < nxy; i += 2) {
-            rd(ldc(d->xy, i + 1), P1);
-            use(ldc(d->xy, i), P0);
-            if (i + 2 < nxy) rd(ldc(d->xy, i + 2), P0);
-            use(ldc(d->xy, i + 1), P1);
-          }
-          if (i < nxy) use(ldc(d->xy, i), P0);
-        } else {
-          Raw P0;
+            for (; i + 1 < i1; i += 2) {
+              rd(ldc(d->xy, i + 1), P1);
+              use(xc, ldc(d->xy, i), P0);
+              if (i + 2 < i1) rd(ldc(d->xy, i + 2), P0);
+              use(xc, ldc(d->xy, i + 1), P1);
+            }
+            if (i < i1) use(xc, ldc(d->xy, i), P0);
+          } else {
+            Raw P0;
 #pragma unroll 1
-          for (int i = 0; i < nxy; ++i) {
-            rd(ldc(d->xy, i), P0);
-            use(ldc(d->xy, i), P0);
+            for (int i = i0; i < i1; ++i) {
+              rd(ldc(d->xy, i), P0);
+              use(xc, ldc(d->xy, i), P0);
+            }
           }
-        }
+        };
+        const int nx = ldc(&d->nx, 0), nxy = ldc(&d->nxy, 0);
+        run(std::true_type{}, 0, nx);
+        run(std::false_type{}, nx, nxy);
         __syncthreads();  // the tile buffer is B's from here on
       }
     }

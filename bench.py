@@ -114,10 +114,12 @@ def bench_qaoa(args, sync):
     from qradient_amd.batch import QaoaBatch
     from qradient_amd.optimization_problems import MaxCut
     n, p, B = 20, 8, 512
-    rs = np.random.RandomState(SEED)
-    obs = [MaxCut.regular3(n, rs).to_observable() for _ in range(B)]
-    gammas = np.pi * rs.rand(B, p)
-    betas = np.pi * rs.rand(B, p)
+    # the workload's graphs and angles, committed as a fixture (written by
+    # tests/golden/make_golden.py with the library's seeded generator)
+    wl = np.load(os.path.join(ROOT, 'tests', 'golden', 'qaoa_reg3_n20_b512.npz'), allow_pickle=False)
+    obs = [MaxCut(n, edge_set=wl['edges'][b]).to_observable() for b in range(B)]
+    gammas = wl['gammas']
+    betas = wl['betas']
     res = {}
     for dtype in ('complex64', 'complex128'):
         qb = QaoaBatch(n, p, B, dtype=dtype)
@@ -144,7 +146,7 @@ def bench_qaoa(args, sync):
                  'reductions)', B * 1e3 / c64['ms'], 'circuits/s', c64['steps'], args.warmup, c64['ms'],
                  'c64 (FP64 arithmetic and reductions)',
                  {'workload': 'qaoa_maxcut_reg3_n20_p8_batch512', 'qubits': n, 'layers': p, 'batch': B,
-                  'seed': SEED},
+                  'seed': SEED, 'graphs': 'tests/golden/qaoa_reg3_n20_b512.npz'},
                  device_ms_per_step=c64['dev_ms'],
                  c128_plain={'circuits_per_s': B * 1e3 / c128['ms'], 'ms_per_step': c128['ms']},
                  c64_vs_c128_grad_rel_err_max=float(rel.max()),

@@ -1,0 +1,64 @@
+"""The golden fixtures against the oracle and the input generators (CPU).
+
+The fixtures were written by tests/golden/make_golden.py from the oracle; these
+tests recompute them, with the matrix-free restatement and with the
+reference's own formulation (CSR operators, stored history), so a drift of
+either shows up here.
+"""
+import numpy as np
+
+from oracle import reference_algorithm as ra
+from oracle import statevector as sv
+from tests.golden import fixtures as fx
+
+TOL = 1e-13
+
+
+def _rel(a, b):
+    return np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300)
+
+
+def test_mcclean_config1_fixture():
+    d = fx.load('mcclean_n10_L10_z0z1.npz')
+    obs = fx.obs_from(d)
+    for ent in ('cnot', 'cz'):
+        E, g = sv.mcclean_grad_run_adjoint(10, d['axes'], d['angles'], obs, entangler=ent)
+        assert abs(E - d['E_' + ent]) <= TOL and _rel(g, d['grad_' + ent]) <= TOL
+    ref = ra.McCleanRef(10, obs, d['axes'], d['angles'])
+    ref.build()
+    E, g = ref.grad_run()
+    assert abs(E - d['E_cnot']) <= TOL and _rel(g, d['grad_cnot']) <= TOL
+
+
+def test_mcclean_config1_inputs_follow_the_reference_draw_order():
+    d = fx.load('mcclean_n10_L10_z0z1.npz')
+    axes, angles = sv.mcclean_random_params(10, 10, np.random.RandomState(1234))
+    assert np.array_equal(axes, d['axes']) and np.array_equal(angles, d['angles'])
+
+
+def test_mcclean_n14_fixture():
+    d = fx.load('mcclean_n14_L3_zzchain.npz')
+    E, g = sv.mcclean_grad_run_adjoint(14, d['axes'], d['angles'], fx.obs_from(d))
+    assert abs(E - d['E']) <= TOL * max(1.0, abs(d['E'])) and _rel(g, d['grad']) <= TOL
+
+
+def test_qaoa_small_fixture():
+    d = fx.load('qaoa_n8_p3_reg3.npz')
+    obs = fx.maxcut_obs(8, d['edges'])
+    E, g = sv.qaoa_grad_run_adjoint(8, obs, d['betas'], d['gammas'])
+    assert abs(E - d['E']) <= TOL * max(1.0, abs(d['E'])) and _rel(g, d['grad']) <= TOL
+
+
+def test_qaoa_batch_workload_fixture():
+    """The 512-graph workload: reproduced by the library's seeded generator,
+    and every graph is simple and 3-regular with edges a < b."""
+    from qradient_amd.optimization_problems import MaxCut
+    d = fx.load('qaoa_reg3_n20_b512.npz')
+    rs = np.random.RandomState(2024)
+    edges = np.array([MaxCut.regular3(20, rs).edge_set for _ in range(512)])
+    assert np.array_equal(edges, d['edges'])
+    assert np.array_equal(np.pi * rs.rand(512, 8), d['gammas'])
+    assert np.array_equal(np.pi * rs.rand(512, 8), d['betas'])
+    for e in d['edges']:
+        assert np.all(e[:, 0] < e[:, 1]) and len({tuple(x) for x in e}) == 30
+        assert np.array_equal(np.bincount(e.ravel(), minlength=20), np.full(20, 3))

@@ -307,6 +307,8 @@ def main():
                      'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic,
                      'kernel': {0: 'forward pass', 1: 'adjoint two-state pass', 2: 'forward + observable pass'}[dom],
                      'avg_launch_ms': avg_ms, 'algorithmic_bytes_per_launch': alg_bytes,
+                     # measured HBM bytes of the kernel over its live launch time
+                     'traffic_gbs': (traffic / (avg_ms * 1e-3) / 1e9) if traffic else None,
                      'ms_per_step_by_kind': totals},
     }
     if not args.no_cpu_baseline:

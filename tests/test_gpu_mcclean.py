@@ -235,3 +235,23 @@ def test_z_terms_on_pass_path(gpu, n, L, dtype, tol):
     Eo, go = sv.mcclean_grad_run_adjoint(n, axes, angles, obs)
     assert abs(E - Eo) <= tol * max(1.0, abs(Eo))
     assert rel_err(g, go) <= tol
+
+
+@pytest.mark.parametrize('n,lds64', [(28, -1), (29, 1)])
+def test_states_of_4_gib_and_more(gpu, n, lds64):
+    """64-bit addressing (states of 4 GiB and more): complex128 at n = 28 and
+    both dtypes at n = 29, checked against each other (the oracle is too slow
+    at this size); complex64 at n = 29 with FP64 tiles."""
+    from qradient_amd.circuit_logic import McClean
+    L = 1
+    axes, angles, obs = _case(n, L, 130 + n)
+    c128 = McClean(n, obs, L, axes=axes, angles=angles)
+    E1, g1 = c128.grad_run()
+    c128._destroy_plan()
+    c64 = McClean(n, obs, L, axes=axes, angles=angles, dtype='complex64')
+    _lds64(c64, lds64)
+    E2, g2 = c64.grad_run()
+    c64._destroy_plan()
+    assert abs(E1 - E2) <= C64_TOL * max(1.0, abs(E1))
+    assert rel_err(g2, g1) <= C64_TOL
+    assert np.all(np.isfinite(g1)) and np.linalg.norm(g1) > 0

@@ -109,6 +109,15 @@ int qg_qaoa_grad_run(qg_plan* plan, int B, const qg_term* terms, const int32_t* 
 /* Qaoa.run_expec_val with exact_expec_val=True (qaoa.py:23-38). */
 int qg_qaoa_expec_val(qg_plan* plan, int B, const qg_term* terms, const int32_t* term_off, const double* betas,
                       const double* gammas, double* E_out);
+/* As qg_qaoa_grad_run / qg_qaoa_expec_val, from given initial states instead
+ * of |+>^n (qaoa.py:40-49 with ini_state): ini_states holds B states of 2^n
+ * complex128 amplitudes (interleaved re, im), circuit b at b * 2^(n+1)
+ * doubles.  The states are read, never modified (the reference multiplies its
+ * ini_state in place). */
+int qg_qaoa_grad_run_ini(qg_plan* plan, int B, const qg_term* terms, const int32_t* term_off, const double* betas,
+                         const double* gammas, const double* ini_states, double* E_out, double* grad_out);
+int qg_qaoa_expec_val_ini(qg_plan* plan, int B, const qg_term* terms, const int32_t* term_off, const double* betas,
+                          const double* gammas, const double* ini_states, double* E_out);
 
 int qg_plan_workspace_size(const qg_plan* plan, size_t* bytes);
 /* Hand the plan caller-owned device memory (e.g. a torch tensor) of at least

@@ -49,6 +49,8 @@ SIGNATURES = {
     'qg_qaoa_plan': [_vp, _i, _i, _i, _i, _pp],
     'qg_qaoa_grad_run': [_vp, _i, _tp, _ip, _dp, _dp, _dp, _dp],
     'qg_qaoa_expec_val': [_vp, _i, _tp, _ip, _dp, _dp, _dp],
+    'qg_qaoa_grad_run_ini': [_vp, _i, _tp, _ip, _dp, _dp, _dp, _dp, _dp],
+    'qg_qaoa_expec_val_ini': [_vp, _i, _tp, _ip, _dp, _dp, _dp, _dp],
     'qg_plan_workspace_size': [_vp, ctypes.POINTER(ctypes.c_size_t)],
     'qg_plan_set_workspace': [_vp, _vp, ctypes.c_size_t],
     'qg_plan_get_state': [_vp, _dp],

@@ -64,8 +64,16 @@ class QaoaBatch:
         g = np.ascontiguousarray(gammas, dtype=np.float64).reshape(B, self.p)
         return b, g
 
-    def grad_run(self, observables, betas, gammas):
-        """(E[B], grad[B, p, 2]) with grad[b, i] = (dE/dbeta_i, dE/dgamma_i) of circuit b."""
+    def _ini(self, ini_states, B):
+        """B initial states as one contiguous complex128 block (read only)."""
+        v = np.ascontiguousarray(ini_states, dtype=np.complex128)
+        if v.shape != (B, 2 ** self.n):
+            raise ValueError('ini_states must have shape ({}, {})'.format(B, 2 ** self.n))
+        return v, v.view(np.float64).ctypes.data_as(_native._dp)
+
+    def grad_run(self, observables, betas, gammas, ini_states=None):
+        """(E[B], grad[B, p, 2]) with grad[b, i] = (dE/dbeta_i, dE/dgamma_i) of circuit b.
+        ini_states: optional (B, 2**n) initial states (default |+>^n per circuit)."""
         B = len(observables)
         if B > self.batch:
             raise ValueError('{} circuits, the plan holds {}'.format(B, self.batch))
@@ -73,17 +81,29 @@ class QaoaBatch:
         b, g = self._params(betas, gammas, B)
         E = np.empty(B)
         grad = np.empty((B, self.p, 2))
-        _native.call('qg_qaoa_grad_run', self._plan, B, arr, _native.iptr(off), _native.dptr(b), _native.dptr(g),
-                     _native.dptr(E), _native.dptr(grad))
+        if ini_states is None:
+            _native.call('qg_qaoa_grad_run', self._plan, B, arr, _native.iptr(off), _native.dptr(b),
+                         _native.dptr(g), _native.dptr(E), _native.dptr(grad))
+        else:
+            _keep, ini = self._ini(ini_states, B)
+            _native.call('qg_qaoa_grad_run_ini', self._plan, B, arr, _native.iptr(off), _native.dptr(b),
+                         _native.dptr(g), ini, _native.dptr(E), _native.dptr(grad))
         return E, grad
 
-    def expec_val(self, observables, betas, gammas):
+    def expec_val(self, observables, betas, gammas, ini_states=None):
         B = len(observables)
+        if B > self.batch:
+            raise ValueError('{} circuits, the plan holds {}'.format(B, self.batch))
         arr, off = self._terms(observables)
         b, g = self._params(betas, gammas, B)
         E = np.empty(B)
-        _native.call('qg_qaoa_expec_val', self._plan, B, arr, _native.iptr(off), _native.dptr(b), _native.dptr(g),
-                     _native.dptr(E))
+        if ini_states is None:
+            _native.call('qg_qaoa_expec_val', self._plan, B, arr, _native.iptr(off), _native.dptr(b),
+                         _native.dptr(g), _native.dptr(E))
+        else:
+            _keep, ini = self._ini(ini_states, B)
+            _native.call('qg_qaoa_expec_val_ini', self._plan, B, arr, _native.iptr(off), _native.dptr(b),
+                         _native.dptr(g), ini, _native.dptr(E))
         return E
 
     def last_device_ms(self):

@@ -56,13 +56,17 @@ class Qaoa(ParametrizedCircuit):
     def grad_run(self, betas, gammas, hide_progbar=True, ini_state=None):
         """(E, grad[p, 2]): qaoa.py:40-70."""
         betas, gammas = self.__check_parameters(betas, gammas)
-        if ini_state is not None:
-            raise NotImplementedError('ini_state for QAOA grad_run is not supported on the GPU path yet')
         arr, off = self._terms()
         E = np.empty(1)
         grad = np.empty((self.lnum, 2), dtype='double')
-        _native.call('qg_qaoa_grad_run', self._get_plan(), 1, arr, _native.iptr(off), _native.dptr(betas),
-                     _native.dptr(gammas), _native.dptr(E), _native.dptr(grad))
+        if ini_state is None:
+            _native.call('qg_qaoa_grad_run', self._get_plan(), 1, arr, _native.iptr(off), _native.dptr(betas),
+                         _native.dptr(gammas), _native.dptr(E), _native.dptr(grad))
+        else:
+            # the library only reads ini_state (the reference scales it in place)
+            _keep, ini = self._ini(ini_state, self.qnum)
+            _native.call('qg_qaoa_grad_run_ini', self._get_plan(), 1, arr, _native.iptr(off), _native.dptr(betas),
+                         _native.dptr(gammas), ini, _native.dptr(E), _native.dptr(grad))
         return np.float64(E[0]), grad
 
     def run_expec_val(self, betas, gammas, hide_progbar=True, exact_expec_val=True, shot_num=1, ini_state=None):
@@ -70,10 +74,13 @@ class Qaoa(ParametrizedCircuit):
         if not exact_expec_val:
             raise NotImplementedError('finite-shot expectation values are outside the GPU grad_run path')
         betas, gammas = self.__check_parameters(betas, gammas)
-        if ini_state is not None:
-            raise NotImplementedError('ini_state for QAOA is not supported on the GPU path yet')
         arr, off = self._terms()
         E = np.empty(1)
-        _native.call('qg_qaoa_expec_val', self._get_plan(), 1, arr, _native.iptr(off), _native.dptr(betas),
-                     _native.dptr(gammas), _native.dptr(E))
+        if ini_state is None:
+            _native.call('qg_qaoa_expec_val', self._get_plan(), 1, arr, _native.iptr(off), _native.dptr(betas),
+                         _native.dptr(gammas), _native.dptr(E))
+        else:
+            _keep, ini = self._ini(ini_state, self.qnum)
+            _native.call('qg_qaoa_expec_val_ini', self._get_plan(), 1, arr, _native.iptr(off), _native.dptr(betas),
+                         _native.dptr(gammas), ini, _native.dptr(E))
         return np.float64(E[0])

@@ -334,6 +334,7 @@ struct qg_plan {
   std::vector<DTerm> h_terms;   // host copy of what d_terms holds for the current run
   std::vector<DTerm> h_cterms;  // the terms split per pass launch (PassDesc::cterms)
   DTerm* d_cterms = nullptr;
+  void* d_ini_states = nullptr;  // QAOA: the chunk's initial states (when given)
   size_t cterm_cap = 0;
   DTerm* d_xy = nullptr;
   int max_terms = 0;
@@ -813,6 +814,8 @@ static void free_small(qg_plan* p) {
   drop_graph(p);  // its launches point into the descriptor table freed here
   (void)hipFree(p->d_cterms);
   p->d_cterms = nullptr;
+  (void)hipFree(p->d_ini_states);
+  p->d_ini_states = nullptr;
   p->cterm_cap = 0;
   (void)hipFree(p->d_map);
   (void)hipFree(p->d_init);
@@ -1405,8 +1408,9 @@ extern "C" int qg_qaoa_plan(qg_ctx* ctx, int n, int p_layers, int dtype, int bat
   return QG_OK;
 }
 
+// ini: B initial states (complex128, 2^n each) or null for |+>^n (qaoa.py:17)
 static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term_off, const double* betas,
-                    const double* gammas, double* E_out, double* grad_out, bool grad) {
+                    const double* gammas, const double* ini, double* E_out, double* grad_out, bool grad) {
   if (!p || p->kind != PK_QAOA) return fail(QG_ERR_INVALID, "not a QAOA plan");
   if (B < 1 || B > p->batch) return fail(QG_ERR_INVALID, "B must be in [1, batch]");
   if (!term_off || !betas || !gammas || !E_out || (grad && !grad_out)) return fail(QG_ERR_INVALID, "NULL argument");
@@ -1437,7 +1441,7 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
   const int Wi = (int)W;
   qg_ctx* c = p->ctx;
   HIPCHK(hipSetDevice(c->device));
-  if (p->fused && n <= kFusedMaxN) {
+  if (p->fused && n <= kFusedMaxN && !ini) {
     const int per = 1 + 2 * L;
     std::vector<double> res((size_t)B * per);
     auto fill = [&](int b, FusedDescH& d, Arena& ar) -> int {
@@ -1466,6 +1470,10 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
   CHK(ensure_workspace(p));
   const int chunk = p->chunk;
   const int lut_len = 2 * Wi + 1;
+  if (ini && !p->d_ini_states) {
+    // one initial state per circuit of a chunk, in the storage type
+    HIPCHK(hipMalloc(&p->d_ini_states, (size_t)chunk * p->S));
+  }
   if (phase == 1) {
     const int need = chunk * 2 * L * lut_len * 2;
     if (need > p->lut_len) {
@@ -1489,6 +1497,21 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
     }
     HIPCHK(hipMemcpy(p->d_terms, tbuf.data(), sizeof(DTerm) * tbuf.size(), hipMemcpyHostToDevice));
     p->h_terms = tbuf;
+    if (ini) {
+      // upload (complex128) and convert to the storage type
+      const size_t N = 1ull << n;
+      double* tmp = nullptr;
+      HIPCHK(hipMalloc(&tmp, N * 16));
+      hipError_t e = hipSuccess;
+      for (int j = 0; j < nb && e == hipSuccess; ++j) {
+        e = hipMemcpyAsync(tmp, ini + (size_t)(c0 + j) * 2 * N, N * 16, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess)
+          e = simple_convert_in(p->dtype, tmp, (char*)p->d_ini_states + (size_t)j * p->S, n, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+      }
+      (void)hipFree(tmp);
+      if (e != hipSuccess) return fail(QG_ERR_HIP, std::string("ini_state upload: ") + hipGetErrorString(e));
+    }
     std::vector<double> lut;
     if (phase == 1) {
       lut.resize((size_t)nb * 2 * L * lut_len * 2);
@@ -1568,7 +1591,7 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
           PassDesc& d = base_desc(cur_geo, j);
           uint32_t f = F_ROT_A | F_STORE_A | phase_flag;
           if (g == 0 && (i == 0 || !phase_last)) {
-            f |= (i == 0 ? F_INIT_TAB : F_LOAD_A) | F_PHASE_PRE;
+            f |= (i == 0 && !ini ? F_INIT_TAB : F_LOAD_A) | F_PHASE_PRE;
             d.phase_gamma = gammas[(c0 + j) * L + i];
             d.lut = lut_ptr(j, i, 0);
           } else {
@@ -1579,7 +1602,8 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
             d.phase_gamma = gammas[(c0 + j) * L + i + 1];
             d.lut = lut_ptr(j, i + 1, 0);
           }
-          d.a_in = (i == 0 && g == 0) ? nullptr : (g == 0 ? hist(j, i - 1, P - 1) : hist(j, i, g - 1));
+          d.a_in = (i == 0 && g == 0) ? (ini ? (const void*)((char*)p->d_ini_states + (size_t)j * p->S) : nullptr)
+                                      : (g == 0 ? hist(j, i - 1, P - 1) : hist(j, i, g - 1));
           d.a_out = hist(j, i, g);
           for (int lb = G.own_lo; lb < G.own_hi; ++lb) set_gate(d, lb, QG_AXIS_X, betas[(c0 + j) * L + i], 1.0);
           if (last) {
@@ -1670,12 +1694,24 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
 
 extern "C" int qg_qaoa_grad_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term_off,
                                 const double* betas, const double* gammas, double* E_out, double* grad_out) {
-  return qaoa_run(p, B, terms, term_off, betas, gammas, E_out, grad_out, true);
+  return qaoa_run(p, B, terms, term_off, betas, gammas, nullptr, E_out, grad_out, true);
+}
+
+extern "C" int qg_qaoa_grad_run_ini(qg_plan* p, int B, const qg_term* terms, const int32_t* term_off,
+                                    const double* betas, const double* gammas, const double* ini_states,
+                                    double* E_out, double* grad_out) {
+  return qaoa_run(p, B, terms, term_off, betas, gammas, ini_states, E_out, grad_out, true);
 }
 
 extern "C" int qg_qaoa_expec_val(qg_plan* p, int B, const qg_term* terms, const int32_t* term_off,
                                  const double* betas, const double* gammas, double* E_out) {
-  return qaoa_run(p, B, terms, term_off, betas, gammas, E_out, nullptr, false);
+  return qaoa_run(p, B, terms, term_off, betas, gammas, nullptr, E_out, nullptr, false);
+}
+
+extern "C" int qg_qaoa_expec_val_ini(qg_plan* p, int B, const qg_term* terms, const int32_t* term_off,
+                                     const double* betas, const double* gammas, const double* ini_states,
+                                     double* E_out) {
+  return qaoa_run(p, B, terms, term_off, betas, gammas, ini_states, E_out, nullptr, false);
 }
 
 // ---------------------------------------------------------------------------

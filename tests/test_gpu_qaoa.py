@@ -103,3 +103,69 @@ def test_qaoa_pass_path_diag_terms(gpu, n, p, integral, dtype, tol):
     Eo, go = sv.qaoa_grad_run(n, obs, b, g)
     assert abs(E - Eo) <= tol * max(1.0, abs(Eo))
     assert rel_err(grad, go) <= tol, rel_err(grad, go)
+
+
+def _random_state(n, rs):
+    v = rs.standard_normal(2 ** n) + 1j * rs.standard_normal(2 ** n)
+    return v / np.linalg.norm(v)
+
+
+@pytest.mark.parametrize('n,p', [(5, 2), (11, 3), (16, 2)])
+@pytest.mark.parametrize('dtype,tol', [('complex128', 1e-12), ('complex64', 1e-6)])
+def test_qaoa_ini_state(gpu, n, p, dtype, tol):
+    """grad_run / run_expec_val from a given initial state (qaoa.py:23-38,
+    40-70); the caller's array is left as it was."""
+    from qradient_amd.circuit_logic import Qaoa
+    rs = np.random.RandomState(60 + n)
+    obs = sv.maxcut_observable(n, _graph(n, n))
+    b, g = np.pi * rs.rand(p), np.pi * rs.rand(p)
+    ini = _random_state(n, rs)
+    keep = ini.copy()
+    circ = Qaoa(n, obs, p, dtype=dtype)
+    E, grad = circ.grad_run(b, g, ini_state=ini)
+    Eo, go = sv.qaoa_grad_run(n, obs, b, g, ini_state=keep)
+    assert abs(E - Eo) <= tol * max(1.0, abs(Eo))
+    assert rel_err(grad, go) <= tol, rel_err(grad, go)
+    assert abs(circ.run_expec_val(b, g, ini_state=ini) - Eo) <= tol * max(1.0, abs(Eo))
+    assert np.array_equal(ini, keep)
+    # the same plan still runs from |+>^n afterwards
+    E2, g2 = circ.grad_run(b, g)
+    Eo2, go2 = sv.qaoa_grad_run(n, obs, b, g)
+    assert abs(E2 - Eo2) <= tol * max(1.0, abs(Eo2)) and rel_err(g2, go2) <= tol
+
+
+def test_qaoa_ini_state_composes_layers(gpu):
+    """Two layers from |+>^n equal one layer from the state after the first:
+    the energy and the second layer's gradient agree."""
+    from qradient_amd.circuit_logic import Qaoa
+    n = 14
+    rs = np.random.RandomState(8)
+    obs = sv.maxcut_observable(n, _graph(n, 8))
+    b, g = np.pi * rs.rand(2), np.pi * rs.rand(2)
+    E2, grad2 = Qaoa(n, obs, 2).grad_run(b, g)
+    C = sv.classical_ham(n, obs)
+    mid = sv.reset_state(n, '+') * np.exp(-1j * g[0] * C)
+    sv.xrot_all(mid, b[0])
+    E1, grad1 = Qaoa(n, obs, 1).grad_run(b[1:], g[1:], ini_state=mid)
+    assert abs(E1 - E2) < 1e-11
+    assert rel_err(grad1[0], grad2[1]) < 1e-11
+
+
+@pytest.mark.parametrize('dtype,tol', [('complex128', 1e-12), ('complex64', 1e-6)])
+def test_batched_ini_states(gpu, dtype, tol):
+    from qradient_amd.batch import QaoaBatch
+    n, p, B = 15, 2, 5
+    rs = np.random.RandomState(12)
+    obs = [sv.maxcut_observable(n, _graph(n, 20 + b, m=2 * n)) for b in range(B)]
+    betas, gammas = np.pi * rs.rand(B, p), np.pi * rs.rand(B, p)
+    ini = np.stack([_random_state(n, rs) for _ in range(B)])
+    keep = ini.copy()
+    batch = QaoaBatch(n, p, B, dtype=dtype)
+    E, grad = batch.grad_run(obs, betas, gammas, ini_states=ini)
+    Ev = batch.expec_val(obs, betas, gammas, ini_states=ini)
+    for k in range(B):
+        Eo, go = sv.qaoa_grad_run(n, obs[k], betas[k], gammas[k], ini_state=keep[k])
+        assert abs(E[k] - Eo) <= tol * max(1.0, abs(Eo))
+        assert abs(Ev[k] - Eo) <= tol * max(1.0, abs(Eo))
+        assert rel_err(grad[k], go) <= tol
+    assert np.array_equal(ini, keep)

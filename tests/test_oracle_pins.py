@@ -327,6 +327,39 @@ def test_qaoa_dense_and_finite_differences():
         assert abs(fdb - grad[i, 0]) < 1e-7 and abs(fdg - grad[i, 1]) < 1e-7
 
 
+def test_qaoa_ini_state_dense_and_finite_differences():
+    """QAOA from a random initial state: dense unitaries for E, central
+    finite differences for both gradient columns; the input is not modified."""
+    n, p = 5, 2
+    edges = [(0, 1), (1, 2), (2, 3), (3, 4), (0, 4)]
+    obs = sv.maxcut_observable(n, edges)
+    rs = np.random.RandomState(3)
+    b, g = np.pi * rs.rand(p), np.pi * rs.rand(p)
+    ini = rs.standard_normal(2 ** n) + 1j * rs.standard_normal(2 ** n)
+    ini /= np.linalg.norm(ini)
+    keep = ini.copy()
+    C = np.zeros(2 ** n)
+    for (u, v) in edges:
+        C += np.diag(op1(n, u, PZ) @ op1(n, v, PZ)).real
+    B = sum(op1(n, q, PX) for q in range(n))
+    psi = ini.copy()
+    for i in range(p):
+        psi = la.expm(-0.5j * b[i] * B) @ (np.exp(-1j * g[i] * C) * psi)
+    E, grad = sv.qaoa_grad_run(n, obs, b, g, ini_state=ini)
+    assert abs(E - np.vdot(psi, C * psi).real) < 1e-12
+    assert abs(sv.qaoa_run_expec_val(n, obs, b, g, ini_state=ini) - E) < 1e-12
+    Ea, ga = sv.qaoa_grad_run_adjoint(n, obs, b, g, ini_state=ini)
+    assert abs(Ea - E) < 1e-12 and np.abs(ga - grad).max() < 1e-12
+    eps = 1e-6
+    for i in range(p):
+        d = np.zeros(p)
+        d[i] = eps
+        fdb = (sv.qaoa_run_expec_val(n, obs, b + d, g, ini) - sv.qaoa_run_expec_val(n, obs, b - d, g, ini)) / (2 * eps)
+        fdg = (sv.qaoa_run_expec_val(n, obs, b, g + d, ini) - sv.qaoa_run_expec_val(n, obs, b, g - d, ini)) / (2 * eps)
+        assert abs(fdb - grad[i, 0]) < 1e-7 and abs(fdg - grad[i, 1]) < 1e-7
+    assert np.array_equal(ini, keep)
+
+
 def test_complex64_model_error_is_bounded():
     """The complex64 model (FP32 gates) lands within 1e-5 of complex128 at small n."""
     n, L = 8, 6

@@ -125,6 +125,17 @@ int qg_plan_workspace_size(const qg_plan* plan, size_t* bytes);
 int qg_plan_set_workspace(qg_plan* plan, void* dev_ptr, size_t bytes);
 /* Copy the final state of the last forward run (complex128 interleaved). */
 int qg_plan_get_state(qg_plan* plan, double* out);
+
+/* prob[k] = |P_k^+ psi|^2 = <psi|(I + P_k)/2|psi>, the probability of the +1
+ * outcome of term k of terms[0, nterms) (weights ignored), in the state of the
+ * plan's last forward run, in the order given.  Replaces the projector
+ * probabilities of ParametrizedCircuit.sample_expec_val
+ * (qradient/circuit_logic/base.py:22-46; Projector at
+ * qradient/physical_components/observable.py:82-104, 126-177); the shots are
+ * drawn by the caller.  Needs n <= 31.  After a batched QAOA run (B > 1) the
+ * state is that of the first circuit of the last chunk on the pass kernels,
+ * and absent after the whole-circuit kernel. */
+int qg_plan_term_prob(qg_plan* plan, const qg_term* terms, int nterms, double* prob);
 /* Device time of the last run in milliseconds (hipEvents around the launches). */
 int qg_plan_last_ms(const qg_plan* plan, double* ms);
 /* Record a HIP event before every kernel launch of later runs (on = 1). */
@@ -188,6 +199,10 @@ int qg_state_expec_val(qg_state* st, const qg_term* terms, int nterms, double* o
 int qg_state_scale(qg_state* st, double re, double im);
 /* 1 - ||vec|| (state.py:331-332) */
 int qg_state_norm_error(qg_state* st, double* out);
+
+/* prob[k] = |P_k^+ vec|^2 for each term (the projector probabilities of
+ * base.py:22-46), as qg_plan_term_prob. */
+int qg_state_term_prob(qg_state* st, const qg_term* terms, int nterms, double* prob);
 
 #ifdef __cplusplus
 }

@@ -272,6 +272,55 @@ def expec_val(v, obs):
 
 
 # --------------------------------------------------------------------------
+# finite-shot expectation values (base.py:22-46, observable.py:82-104, 126-177)
+# --------------------------------------------------------------------------
+_PROJ = {'x': np.array([[.5, .5], [.5, .5]], dtype=complex),      # observable.py:130
+         'y': np.array([[.5, -.5j], [.5j, .5]], dtype=complex)}   # observable.py:137
+
+
+def projector_prob(v, kind, i, j=-1):
+    """|P^+ v|^2 for one term: the +1-eigenspace projector of Projector
+    (observable.py:126-177) applied to v, then the squared norm (base.py:28)."""
+    v = np.asarray(v, dtype=np.complex128)
+    n = int(v.size).bit_length() - 1
+    if kind in ('x', 'y'):
+        w = _view(v, i)
+        P = _PROJ[kind]
+        p0 = P[0, 0] * w[:, 0, :] + P[0, 1] * w[:, 1, :]
+        p1 = P[1, 0] * w[:, 0, :] + P[1, 1] * w[:, 1, :]
+        return float((np.abs(p0) ** 2).sum() + (np.abs(p1) ** 2).sum())
+    zs = z_signs(n)
+    if kind == 'z':
+        keep = zs[i] > 0
+    elif kind == 'zz':
+        keep = zs[i] * zs[j] > 0   # up-up + down-down (observable.py:150-164)
+    else:
+        raise ValueError('Unknown key for projector {}.'.format(kind))
+    return float((np.abs(v[keep]) ** 2).sum())
+
+
+def term_probs(v, obs):
+    """Projector probabilities of every term, in load_projectors order
+    (observable.py:82-104: x, y, z by qubit, then zz row-major)."""
+    _, terms = observable_terms(obs)
+    return np.array([projector_prob(v, k, i, j) for k, i, j, _ in terms]), np.array([t[3] for t in terms])
+
+
+def sample_expec_val(v, obs, shot_num, random_state=None):
+    """ParametrizedCircuit.sample_expec_val (base.py:22-34), with scipy's
+    rv_discrete as the reference calls it (random_state=None: numpy's global
+    RandomState, as there)."""
+    from scipy import stats
+    probs, weights = term_probs(v, obs)
+    total = 0.
+    for prob, weight in zip(probs, weights):
+        rnd_var = stats.rv_discrete(values=([0, 1], [prob, 1 - prob]))
+        samples = np.array([weight, -weight])[rnd_var.rvs(size=shot_num, random_state=random_state)]
+        total += samples.mean()
+    return total
+
+
+# --------------------------------------------------------------------------
 # McClean (mc_clean.py:9-78)
 # --------------------------------------------------------------------------
 def mcclean_random_params(L, n, rng=None):
@@ -421,13 +470,20 @@ def x_summed(v):
     return out
 
 
-def qaoa_run_expec_val(n, obs, betas, gammas, ini_state=None):
-    """Qaoa.run_expec_val with exact expectation (qaoa.py:23-38)."""
+def qaoa_forward(n, obs, betas, gammas, ini_state=None):
+    """The state after Qaoa's forward loop (qaoa.py:29-35)."""
     C = classical_ham(n, obs)
     v = reset_state(n, '+') if ini_state is None else np.array(ini_state, dtype=np.complex128)
     for b, g in zip(betas, gammas):
         v *= np.exp(-1j * g * C)
         xrot_all(v, b)
+    return v
+
+
+def qaoa_run_expec_val(n, obs, betas, gammas, ini_state=None):
+    """Qaoa.run_expec_val with exact expectation (qaoa.py:23-38)."""
+    C = classical_ham(n, obs)
+    v = qaoa_forward(n, obs, betas, gammas, ini_state)
     return float(np.vdot(v, C * v).real)
 
 

@@ -78,6 +78,8 @@ SIGNATURES = {
     'qg_state_multiply_observable': [_vp, _tp, _i],
     'qg_state_expec_val': [_vp, _tp, _i, _dp],
     'qg_state_norm_error': [_vp, _dp],
+    'qg_state_term_prob': [_vp, _tp, _i, _dp],
+    'qg_plan_term_prob': [_vp, _tp, _i, _dp],
     'qg_state_scale': [_vp, _d, _d],
 }
 _RESTYPES = {'qg_last_error': ctypes.c_char_p}

@@ -21,6 +21,32 @@ def progbar_range(hide_progbar):
         return np.arange
 
 
+def sample_projectors(probs, weights, shot_num, random_state=None):
+    """Finite-shot estimate of sum_k w_k <P_k> (base.py:22-34).
+
+    For each term in order: shot_num draws of the two-point distribution
+    P(+w) = prob, P(-w) = 1 - prob, averaged, summed over the terms.  The draws
+    are those of scipy's ``rv_discrete(values=([0, 1], [prob, 1 - prob])).rvs``
+    that the reference makes: one uniform per shot from the legacy global
+    numpy RandomState (or ``random_state``), outcome 0 where it is at most the
+    first cumulative probability.  So a run seeded like the reference's gives
+    the reference's estimate.  A probability that rounding puts just outside
+    [0, 1] is clipped (the reference's rv_discrete would raise).
+    """
+    rs = np.random.mtrand._rand if random_state is None else random_state
+    shot_num = int(shot_num)
+    if shot_num < 1:
+        raise ValueError('shot_num must be at least 1')
+    total = 0.
+    for prob, w in zip(probs, weights):
+        prob = min(max(float(prob), 0.0), 1.0)
+        qvals = np.cumsum([prob, 1. - prob])
+        u = rs.uniform(size=shot_num)
+        idx = np.where(u <= qvals[0], 0, np.where(u <= qvals[1], 1, 0))
+        total += np.array([w, -w])[idx].mean()
+    return total
+
+
 class ParametrizedCircuit:
     """Parent class for VQE circuits. Not meant for instantiation."""
 
@@ -39,6 +65,7 @@ class ParametrizedCircuit:
         self.graph = bool(backend.get('graph', False))
         self.observable = Observable(self.qnum, observable, store_components=use_observable_components)
         self._state = None
+        self._plan_state = False  # the last run_expec_val's final state is the current state
         self._gates = None
         self._plan = None
         self._ctx = None
@@ -51,16 +78,25 @@ class ParametrizedCircuit:
     # The reference allocates State(n) eagerly (base.py:11); here the device
     # state is created on first access so grad_run (which works in plan
     # workspaces) does not hold an unused 2^n vector.
+    # After run_expec_val the final state stays in the plan's workspace; it is
+    # copied into ``state`` only when ``state`` is read (the reference's
+    # run_expec_val leaves it in self.state, mc_clean.py:27-45).
     @property
     def state(self):
         if self._state is None:
             self._state = State(self.qnum, dtype=self.dtype, device=self.device)
             self._state.gates = self._gates
             self._init_state(self._state)
+        if self._plan_state:
+            self._plan_state = False
+            out = np.empty(2 ** self.qnum, dtype=np.complex128)
+            _native.call('qg_plan_get_state', self._plan, out.view(np.float64).ctypes.data_as(_native._dp))
+            self._state.vec = out
         return self._state
 
     @state.setter
     def state(self, value):
+        self._plan_state = False
         self._state = value
 
     def _init_state(self, st):
@@ -75,8 +111,28 @@ class ParametrizedCircuit:
         """<vec|O|vec> of the current state (base.py:17-20), FP64 on the device."""
         return self.state.expec_val(self.observable)
 
-    def sample_expec_val(self, shot_num):
-        raise NotImplementedError('finite-shot sampling is outside the GPU grad_run path')
+    def _term_probs(self, terms):
+        """Projector probabilities of the terms in the current state."""
+        arr, k = _native.terms_array(terms)
+        out = np.empty(k)
+        if k:
+            if self._plan_state:
+                _native.call('qg_plan_term_prob', self._plan, arr, k, _native.dptr(out))
+            else:
+                _native.call('qg_state_term_prob', self.state._h, arr, k, _native.dptr(out))
+        return out
+
+    def sample_expec_val(self, shot_num, random_state=None):
+        """Finite-shot expectation value of the observable in the current state
+        (base.py:22-34): projector probabilities on the device, shot_num shots
+        per term drawn as by the reference."""
+        terms = self.observable.terms
+        return sample_projectors(self._term_probs(terms), [t[3] for t in terms], shot_num, random_state)
+
+    def sample_component_expec_val(self, shot_num, component, random_state=None):
+        """Finite-shot expectation value of one observable term (base.py:36-46)."""
+        t = self.observable.terms[component]
+        return sample_projectors(self._term_probs([t]), [t[3]], shot_num, random_state)
 
     def _destroy_plan(self):
         if self._plan is not None and _native._lib is not None:

@@ -76,18 +76,24 @@ class McClean(ParametrizedCircuit):
         grad = np.empty((self.lnum, self.qnum), dtype='double')
         _native.call('qg_mcclean_grad_run', self._get_plan(), _native.iptr(axes), _native.dptr(angles), arr, k,
                      ini, ctypes.byref(E), _native.dptr(grad))
+        self._plan_state = False
         return np.float64(E.value), grad
 
     def run_expec_val(self, hide_progbar=True, exact_expec_val=True, shot_num=1, ini_state=None):
-        """Forward circuit and exact expectation (mc_clean.py:27-45)."""
-        if not exact_expec_val:
-            raise NotImplementedError('finite-shot expectation values are outside the GPU grad_run path')
+        """Forward circuit, then the exact expectation or, with
+        exact_expec_val=False, its shot_num-shot estimate (mc_clean.py:27-45)."""
         axes, angles = self._params()
         _keep, ini = self._ini(ini_state, self.qnum)
-        arr, k = self.observable.c_terms()
+        # the shot estimate needs only the final state: its forward run takes
+        # the z / zz terms, so x / y terms are sampled at every size
+        arr, k = self.observable.c_terms() if exact_expec_val else _native.terms_array(
+            self.observable.classical_terms())
         E = ctypes.c_double()
         _native.call('qg_mcclean_expec_val', self._get_plan(), _native.iptr(axes), _native.dptr(angles), arr, k,
                      ini, ctypes.byref(E))
+        self._plan_state = True
+        if not exact_expec_val:
+            return self.sample_expec_val(shot_num)
         return np.float64(E.value)
 
     def final_state(self):

@@ -67,12 +67,12 @@ class Qaoa(ParametrizedCircuit):
             _keep, ini = self._ini(ini_state, self.qnum)
             _native.call('qg_qaoa_grad_run_ini', self._get_plan(), 1, arr, _native.iptr(off), _native.dptr(betas),
                          _native.dptr(gammas), ini, _native.dptr(E), _native.dptr(grad))
+        self._plan_state = False
         return np.float64(E[0]), grad
 
     def run_expec_val(self, betas, gammas, hide_progbar=True, exact_expec_val=True, shot_num=1, ini_state=None):
-        """Forward circuit and exact expectation (qaoa.py:23-38)."""
-        if not exact_expec_val:
-            raise NotImplementedError('finite-shot expectation values are outside the GPU grad_run path')
+        """Forward circuit, then the exact expectation or, with
+        exact_expec_val=False, its shot_num-shot estimate (qaoa.py:23-38)."""
         betas, gammas = self.__check_parameters(betas, gammas)
         arr, off = self._terms()
         E = np.empty(1)
@@ -83,4 +83,7 @@ class Qaoa(ParametrizedCircuit):
             _keep, ini = self._ini(ini_state, self.qnum)
             _native.call('qg_qaoa_expec_val_ini', self._get_plan(), 1, arr, _native.iptr(off), _native.dptr(betas),
                          _native.dptr(gammas), ini, _native.dptr(E))
+        self._plan_state = True
+        if not exact_expec_val:
+            return self.sample_expec_val(shot_num)
         return np.float64(E[0])

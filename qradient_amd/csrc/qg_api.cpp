@@ -31,6 +31,8 @@ hipError_t simple_init(int dtype, void* v, int n, const double* tab, hipStream_t
 hipError_t simple_scale(int dtype, void* v, int n, double re, double im, hipStream_t st);
 hipError_t simple_convert_in(int dtype, const void* in_c128, void* out, int n, hipStream_t st);
 hipError_t simple_convert_out(int dtype, const void* in, void* out_c128, int n, hipStream_t st);
+hipError_t simple_term_expec(int dtype, const void* v, int n, const PTerm* t, int nt, double* part, double* out,
+                             hipStream_t st);
 constexpr int kDotBlocks = 512;
 hipError_t launch_fused(const void* descs, int batch, int n, hipStream_t st);
 hipError_t set_fused_attributes();
@@ -500,6 +502,62 @@ extern "C" int qg_plan_last_ms(const qg_plan* p, double* ms) {
   if (!p || !ms) return fail(QG_ERR_INVALID, "NULL argument");
   *ms = p->last_ms;
   return QG_OK;
+}
+
+// prob[k] = <v|(I + P_k)/2|v> = |P_k^+ v|^2 for each term, in the given order
+// (weights ignored): the probability of the +1 outcome that the projectors of
+// observable.py:126-177 give in base.py:22-46, also for a state whose norm is
+// not exactly 1.  One pass over v per kTermsPerLaunch terms.
+static int term_prob(qg_ctx* c, int dtype, const void* v, int n, const qg_term* terms, int nterms, double* prob) {
+  if (nterms < 0 || (nterms > 0 && (!terms || !prob))) return fail(QG_ERR_INVALID, "NULL argument");
+  if (n > 31) return fail(QG_ERR_UNSUPPORTED, "term probabilities need n <= 31");
+  if (nterms == 0) return QG_OK;
+  std::vector<PTerm> h(nterms + 1);
+  for (int k = 0; k < nterms; ++k) {
+    const qg_term& t = terms[k];
+    if (t.i < 0 || t.i >= n) return fail(QG_ERR_INVALID, "observable: qubit index out of range");
+    PTerm& q = h[k];
+    q.a = n - 1 - t.i;
+    q.b = -1;
+    q.pad = 0;
+    switch (t.kind) {
+      case QG_TERM_Z: q.kind = 0; break;
+      case QG_TERM_ZZ:
+        if (t.j <= t.i || t.j >= n)
+          return fail(QG_ERR_INVALID, "observable: zz terms need i < j < n (upper triangle, observable.py:56-64)");
+        q.kind = 0;
+        q.b = n - 1 - t.j;
+        break;
+      case QG_TERM_X: q.kind = 1; break;
+      case QG_TERM_Y: q.kind = 2; break;
+      default: return fail(QG_ERR_INVALID, "observable: unknown term kind");
+    }
+  }
+  h[nterms] = PTerm{3, 0, -1, 0};  // |v|^2
+  const int nt = nterms + 1;
+  HIPCHK(hipSetDevice(c->device));
+  const size_t tb = ((sizeof(PTerm) * nt + 255) / 256) * 256;
+  const size_t pb = sizeof(double) * kTermBlocks * kTermsPerLaunch;
+  char* buf = nullptr;
+  HIPCHK(hipMalloc(&buf, tb + pb + sizeof(double) * nt));
+  PTerm* d_t = (PTerm*)buf;
+  double* d_part = (double*)(buf + tb);
+  double* d_out = (double*)(buf + tb + pb);
+  std::vector<double> e(nt);
+  hipError_t err = hipMemcpyAsync(d_t, h.data(), sizeof(PTerm) * nt, hipMemcpyHostToDevice, c->stream);
+  if (err == hipSuccess) err = simple_term_expec(dtype, v, n, d_t, nt, d_part, d_out, c->stream);
+  if (err == hipSuccess) err = hipMemcpyAsync(e.data(), d_out, sizeof(double) * nt, hipMemcpyDeviceToHost, c->stream);
+  if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
+  (void)hipFree(buf);
+  if (err != hipSuccess) return fail(QG_ERR_HIP, std::string("term probabilities: ") + hipGetErrorString(err));
+  for (int k = 0; k < nterms; ++k) prob[k] = 0.5 * (e[nterms] + e[k]);
+  return QG_OK;
+}
+
+extern "C" int qg_plan_term_prob(qg_plan* p, const qg_term* terms, int nterms, double* prob) {
+  if (!p) return fail(QG_ERR_INVALID, "NULL plan");
+  if (!p->last_state_ptr) return fail(QG_ERR_INVALID, "no state: run the circuit first");
+  return term_prob(p->ctx, p->dtype, p->last_state_ptr, p->n, terms, nterms, prob);
 }
 
 extern "C" int qg_plan_get_state(qg_plan* p, double* out) {
@@ -1458,13 +1516,16 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
     size_t mx = 0;
     for (int b = 0; b < B; ++b) mx = std::max(mx, obs[b].diag.size());
     const size_t bytes = (size_t)L * 16 + (mx + 2) * sizeof(DTerm);
-    CHK(run_fused(p, B, bytes, per, grad, nullptr, fill, res.data()));
+    // a single circuit keeps its final state (qg_plan_get_state, qg_plan_term_prob)
+    if (B == 1 && !p->d_fstate) HIPCHK(hipMalloc(&p->d_fstate, ((size_t)16) << n));
+    CHK(run_fused(p, B, bytes, per, grad, B == 1 ? p->d_fstate : nullptr, fill, res.data()));
     for (int b = 0; b < B; ++b) {
       E_out[b] = res[(size_t)b * per];
       if (grad)
         for (int k = 0; k < 2 * L; ++k) grad_out[(size_t)b * 2 * L + k] = res[(size_t)b * per + 1 + k];
     }
-    p->last_state_ptr = nullptr;
+    p->last_state_ptr = B == 1 ? p->d_fstate : nullptr;
+    p->state_sign = false;
     return QG_OK;
   }
   CHK(ensure_workspace(p));
@@ -1763,7 +1824,7 @@ extern "C" int qg_state_destroy(qg_state* s) {
 
 extern "C" int qg_state_create(qg_ctx* ctx, int n, int dtype, char ini, qg_state** out) {
   if (!ctx || !out) return fail(QG_ERR_INVALID, "NULL argument");
-  if (n < 1 || n > kMaxQubits) return fail(QG_ERR_INVALID, "qubit number must be in [1, 32]");
+  if (n < 1 || n > 31) return fail(QG_ERR_INVALID, "State: qubit number must be in [1, 31]");
   if (dtype != QG_C128 && dtype != QG_C64) return fail(QG_ERR_INVALID, "bad dtype");
   if (ini != '0' && ini != '+') return fail(QG_ERR_INVALID, std::string("Invalid initialization format ") + ini + ".");
   HIPCHK(hipSetDevice(ctx->device));
@@ -1934,6 +1995,11 @@ extern "C" int qg_state_expec_val(qg_state* s, const qg_term* terms, int nterms,
   HIPCHK(hipMemcpyAsync(out, s->d_dot + kDotBlocks, sizeof(double), hipMemcpyDeviceToHost, s->ctx->stream));
   HIPCHK(hipStreamSynchronize(s->ctx->stream));
   return QG_OK;
+}
+
+extern "C" int qg_state_term_prob(qg_state* s, const qg_term* terms, int nterms, double* prob) {
+  if (!s) return fail(QG_ERR_INVALID, "NULL state");
+  return term_prob(s->ctx, s->dtype, s->v, s->n, terms, nterms, prob);
 }
 
 extern "C" int qg_state_norm_error(qg_state* s, double* out) {

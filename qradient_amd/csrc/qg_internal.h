@@ -51,6 +51,15 @@ struct DTerm {
   double w;
 };
 
+// One Pauli term whose expectation value <P> is measured (finite-shot
+// sampling): kind 0 = Z_a (b < 0) or Z_a Z_b, 1 = X_a, 2 = Y_a, 3 = the
+// identity (the squared norm); a, b are global bit positions.
+struct PTerm {
+  int32_t kind, a, b, pad;
+};
+constexpr int kTermsPerLaunch = 16;  // terms measured per pass over the state
+constexpr int kTermBlocks = 2048;    // workgroups of a term pass (partials per term)
+
 struct PassDesc {
   // geometry: local bits = [0, c) U [p, p + K - c); tile bits = the rest, in order
   int32_t n, K, t, c, p;

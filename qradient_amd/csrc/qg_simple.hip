@@ -129,6 +129,69 @@ __global__ void k_dot(const typename V2<S>::T* __restrict__ a, const typename V2
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// part[k * gridDim.x + block] = this block's share of <P_k> for the terms
+// t[0, nt), nt <= kTermsPerLaunch, in FP64:
+//   Z / ZZ: |v_x|^2 z(x);  X: Re(conj(v_x) v_{x^m});  Y: Re(conj(v_x) (Y v)_x);
+//   identity (kind 3): |v_x|^2
+// with (Y v)_x = -i v_{x^m} on bit 0 and +i v_{x^m} on bit 1.  Fixed order:
+// grid-stride sums, wave butterflies, then the block's waves in turn.
+template <class S>
+__global__ void __launch_bounds__(256) k_term_expec(const typename V2<S>::T* __restrict__ v, uint32_t N,
+                                                    const PTerm* __restrict__ t, int nt, double* __restrict__ part) {
+  __shared__ double red[4][kTermsPerLaunch];
+  double acc[kTermsPerLaunch];
+#pragma unroll
+  for (int k = 0; k < kTermsPerLaunch; ++k) acc[k] = 0.0;
+  for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N; x += gridDim.x * blockDim.x) {
+    const double ar = v[x].x, ai = v[x].y;
+    const double pr = ar * ar + ai * ai;
+#pragma unroll
+    for (int k = 0; k < kTermsPerLaunch; ++k) {
+      if (k < nt) {
+        const PTerm tk = t[k];
+        if (tk.kind == 3) {
+          acc[k] += pr;
+        } else if (tk.kind == 0) {
+          uint32_t par = (x >> tk.a) & 1u;
+          if (tk.b >= 0) par ^= (x >> tk.b) & 1u;
+          acc[k] += par ? -pr : pr;
+        } else {
+          const uint32_t bit = (x >> tk.a) & 1u;
+          const uint32_t y = x ^ (1u << tk.a);
+          const double cr = v[y].x, ci = v[y].y;
+          if (tk.kind == 1) {
+            acc[k] += ar * cr + ai * ci;
+          } else {
+            const double im = ar * ci - ai * cr;  // Im(conj(v_x) v_{x^m})
+            acc[k] += bit ? -im : im;
+          }
+        }
+      }
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < kTermsPerLaunch; ++k) {
+    double s = acc[k];
+    for (int m = 32; m > 0; m >>= 1) s += __shfl_xor(s, m, 64);
+    if (lane == 0) red[wave][k] = s;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < nt) {
+    const int k = threadIdx.x;
+    part[(size_t)k * gridDim.x + blockIdx.x] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+  }
+}
+
+// out[k] = sum of part[k * np + 0 .. np), one thread per term, in order.
+__global__ void k_sum_terms(const double* __restrict__ part, int np, int nt, double* __restrict__ out) {
+  const int k = threadIdx.x;
+  if (blockIdx.x != 0 || k >= nt) return;
+  double s = 0.0;
+  for (int i = 0; i < np; ++i) s += part[(size_t)k * np + i];
+  out[k] = s;
+}
+
 __global__ void k_sum(const double* __restrict__ part, int np, double* __restrict__ out) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     double s = 0.0;
@@ -172,6 +235,9 @@ __global__ void k_scale(typename V2<S>::T* v, uint32_t N, double re, double im) 
 }
 
 // ---------------------------------------------------------------------------
+// The elementwise kernels count amplitudes in 32 bits: states of up to 2^31.
+constexpr int kSimpleMaxN = 31;
+
 static inline dim3 grid_for(uint64_t n, int threads) { return dim3((unsigned)((n + threads - 1) / threads)); }
 
 #define QG_DT_SWITCH(dtype, CALL) \
@@ -186,6 +252,7 @@ static inline dim3 grid_for(uint64_t n, int threads) { return dim3((unsigned)((n
   } while (0)
 
 hipError_t simple_rot1(int dtype, void* v, int n, int b, int type, double c, double s, hipStream_t st) {
+  if (n < 1 || n > kSimpleMaxN) return hipErrorInvalidValue;
   const uint32_t np = (uint32_t)(1ull << (n - 1));
   QG_DT_SWITCH(dtype, hipLaunchKernelGGL(k_rot1<S>, grid_for(np, 256), dim3(256), 0, st,
                                          (typename V2<S>::T*)v, np, b, type, c, s));
@@ -193,6 +260,7 @@ hipError_t simple_rot1(int dtype, void* v, int n, int b, int type, double c, dou
 }
 
 hipError_t simple_permute(int dtype, const void* in, void* out, int n, const uint32_t* dmap, hipStream_t st) {
+  if (n < 1 || n > kSimpleMaxN) return hipErrorInvalidValue;
   const uint32_t N = (uint32_t)(1ull << n);
   QG_DT_SWITCH(dtype, hipLaunchKernelGGL(k_permute<S>, grid_for(N, 256), dim3(256), 0, st,
                                          (const typename V2<S>::T*)in, (typename V2<S>::T*)out, N, dmap, n));
@@ -201,6 +269,7 @@ hipError_t simple_permute(int dtype, const void* in, void* out, int n, const uin
 
 hipError_t simple_diag_phase(int dtype, void* v, int n, const DTerm* dt, int nt, double angle, int cz,
                              hipStream_t st) {
+  if (n < 1 || n > kSimpleMaxN) return hipErrorInvalidValue;
   const uint32_t N = (uint32_t)(1ull << n);
   QG_DT_SWITCH(dtype, hipLaunchKernelGGL(k_diag_phase<S>, grid_for(N, 256), dim3(256), 0, st,
                                          (typename V2<S>::T*)v, N, dt, nt, angle, cz, n));
@@ -209,6 +278,7 @@ hipError_t simple_diag_phase(int dtype, void* v, int n, const DTerm* dt, int nt,
 
 hipError_t simple_obs_apply(int dtype, const void* in, void* out, int n, const DTerm* dt, int ndt, const DTerm* xy,
                             int nxy, hipStream_t st) {
+  if (n < 1 || n > kSimpleMaxN) return hipErrorInvalidValue;
   const uint32_t N = (uint32_t)(1ull << n);
   QG_DT_SWITCH(dtype, hipLaunchKernelGGL(k_obs_apply<S>, grid_for(N, 256), dim3(256), 0, st,
                                          (const typename V2<S>::T*)in, (typename V2<S>::T*)out, N, dt, ndt, xy, nxy));
@@ -219,6 +289,7 @@ hipError_t simple_obs_apply(int dtype, const void* in, void* out, int n, const D
 constexpr int kDotBlocks = 512;
 hipError_t simple_dot(int dtype, const void* a, const void* b, int n, int mode, double* part, double* out,
                       hipStream_t st) {
+  if (n < 1 || n > kSimpleMaxN) return hipErrorInvalidValue;
   const uint32_t N = (uint32_t)(1ull << n);
   const int blocks = (int)((N + 255) / 256) < kDotBlocks ? (int)((N + 255) / 256) : kDotBlocks;
   QG_DT_SWITCH(dtype, hipLaunchKernelGGL(k_dot<S>, dim3(blocks), dim3(256), 0, st, (const typename V2<S>::T*)a,
@@ -227,19 +298,38 @@ hipError_t simple_dot(int dtype, const void* a, const void* b, int n, int mode, 
   return hipGetLastError();
 }
 
+// out[k] = <P_k> for the nt terms t (device arrays); part needs
+// kTermBlocks * kTermsPerLaunch doubles.
+hipError_t simple_term_expec(int dtype, const void* v, int n, const PTerm* t, int nt, double* part, double* out,
+                             hipStream_t st) {
+  if (n < 1 || n > kSimpleMaxN) return hipErrorInvalidValue;
+  const uint32_t N = (uint32_t)(1ull << n);
+  const int blocks = (int)((N + 255) / 256) < kTermBlocks ? (int)((N + 255) / 256) : kTermBlocks;
+  for (int k0 = 0; k0 < nt; k0 += kTermsPerLaunch) {
+    const int m = nt - k0 < kTermsPerLaunch ? nt - k0 : kTermsPerLaunch;
+    QG_DT_SWITCH(dtype, hipLaunchKernelGGL(k_term_expec<S>, dim3(blocks), dim3(256), 0, st,
+                                           (const typename V2<S>::T*)v, N, t + k0, m, part));
+    hipLaunchKernelGGL(k_sum_terms, dim3(1), dim3(kTermsPerLaunch), 0, st, (const double*)part, blocks, m, out + k0);
+  }
+  return hipGetLastError();
+}
+
 hipError_t simple_init(int dtype, void* v, int n, const double* tab, hipStream_t st) {
+  if (n < 1 || n > kSimpleMaxN) return hipErrorInvalidValue;
   const uint32_t N = (uint32_t)(1ull << n);
   QG_DT_SWITCH(dtype, hipLaunchKernelGGL(k_init<S>, grid_for(N, 256), dim3(256), 0, st, (typename V2<S>::T*)v, N, tab));
   return hipGetLastError();
 }
 
 hipError_t simple_scale(int dtype, void* v, int n, double re, double im, hipStream_t st) {
+  if (n < 1 || n > kSimpleMaxN) return hipErrorInvalidValue;
   const uint32_t N = (uint32_t)(1ull << n);
   QG_DT_SWITCH(dtype, hipLaunchKernelGGL(k_scale<S>, grid_for(N, 256), dim3(256), 0, st, (typename V2<S>::T*)v, N, re, im));
   return hipGetLastError();
 }
 
 hipError_t simple_convert_in(int dtype, const void* in_c128, void* out, int n, hipStream_t st) {
+  if (n < 1 || n > kSimpleMaxN) return hipErrorInvalidValue;
   const uint32_t N = (uint32_t)(1ull << n);
   QG_DT_SWITCH(dtype, hipLaunchKernelGGL(k_convert_in<S>, grid_for(N, 256), dim3(256), 0, st, (const double2*)in_c128,
                                          (typename V2<S>::T*)out, N));
@@ -247,6 +337,7 @@ hipError_t simple_convert_in(int dtype, const void* in_c128, void* out, int n, h
 }
 
 hipError_t simple_convert_out(int dtype, const void* in, void* out_c128, int n, hipStream_t st) {
+  if (n < 1 || n > kSimpleMaxN) return hipErrorInvalidValue;
   const uint32_t N = (uint32_t)(1ull << n);
   QG_DT_SWITCH(dtype, hipLaunchKernelGGL(k_convert_out<S>, grid_for(N, 256), dim3(256), 0, st,
                                          (const typename V2<S>::T*)in, (double2*)out_c128, N));

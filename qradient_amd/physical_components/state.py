@@ -124,6 +124,15 @@ class State:
         _native.call('qg_state_expec_val', self._h, arr, k, ctypes.byref(out))
         return out.value
 
+    def term_probs(self, observable):
+        """|P_k^+ vec|^2 for each term of the observable, in its term order (the
+        projector probabilities of base.py:22-46; FP64 reductions on the device)."""
+        arr, k = observable.c_terms()
+        out = np.empty(k)
+        if k:
+            _native.call('qg_state_term_prob', self._h, arr, k, _native.dptr(out))
+        return out
+
     def _classical(self):
         if self.gates is None or getattr(self.gates, '_classical_terms', None) is None:
             raise AttributeError('classical Hamiltonian not loaded: use Gates(n).add_classical_ham(observable)')

@@ -360,6 +360,47 @@ def test_qaoa_ini_state_dense_and_finite_differences():
     assert np.array_equal(ini, keep)
 
 
+@pytest.mark.parametrize('seed', [0, 1])
+def test_projector_probs_are_dense_expectations(seed):
+    """|P^+ v|^2 = (|v|^2 + <v|P|v>) / 2 with <v|P|v> from dense Kronecker
+    products, for every term kind, also for a state of norm != 1."""
+    n = 4
+    rs = np.random.RandomState(seed)
+    v = (rs.standard_normal(2 ** n) + 1j * rs.standard_normal(2 ** n)) * 0.7
+    nrm = np.vdot(v, v).real
+    P = {'x': PX, 'y': PY, 'z': PZ}
+    for q in range(n):
+        for kind in ('x', 'y', 'z'):
+            e = np.vdot(v, op1(n, q, P[kind]) @ v).real
+            assert abs(sv.projector_prob(v, kind, q) - 0.5 * (nrm + e)) < 1e-12
+        for j in range(q + 1, n):
+            e = np.vdot(v, op1(n, q, PZ) @ op1(n, j, PZ) @ v).real
+            assert abs(sv.projector_prob(v, 'zz', q, j) - 0.5 * (nrm + e)) < 1e-12
+
+
+def test_projector_probs_special_states():
+    s = 2 ** -0.5
+    assert abs(sv.projector_prob(np.array([s, 1j * s]), 'y', 0) - 1.0) < 1e-15    # |+i>
+    assert abs(sv.projector_prob(np.array([s, -s]), 'x', 0)) < 1e-15              # |->
+    assert sv.projector_prob(np.array([0, 0, 1, 0], dtype=complex), 'z', 0) == 0.0  # |10>: qubit 0 is 1
+    assert sv.projector_prob(np.array([0, 0, 1, 0], dtype=complex), 'z', 1) == 1.0
+    assert sv.projector_prob(np.array([0, 0, 0, 1], dtype=complex), 'zz', 0, 1) == 1.0
+
+
+def test_sampled_expectation_converges():
+    """The shot estimate is unbiased: many shots land within 5 sigma of <O>."""
+    n = 3
+    rs = np.random.RandomState(4)
+    v = rs.standard_normal(8) + 1j * rs.standard_normal(8)
+    v /= np.linalg.norm(v)
+    obs = {'x': np.array([0.5, None, None], dtype=object), 'z': np.array([None, -1.0, 0.25], dtype=object)}
+    shots = 200000
+    est = sv.sample_expec_val(v, obs, shots, np.random.RandomState(9))
+    probs, w = sv.term_probs(v, obs)
+    sigma = np.sqrt(np.sum(4 * w ** 2 * probs * (1 - probs) / shots))
+    assert abs(est - sv.expec_val(v, obs)) < 5 * sigma
+
+
 def test_complex64_model_error_is_bounded():
     """The complex64 model (FP32 gates) lands within 1e-5 of complex128 at small n."""
     n, L = 8, 6

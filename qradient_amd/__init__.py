@@ -9,6 +9,6 @@ qradient/physical_components/__init__.py:1-2, qradient/circuit_logic/__init__.py
 All statevector arithmetic runs in HIP kernels on gfx950 through the C ABI in
 include/qradient_amd.h (libqradient_amd.so, bound with ctypes).
 """
-from . import physical_components, circuit_logic, optimization_problems  # noqa: F401
+from . import physical_components, circuit_logic, optimization_problems, optimization  # noqa: F401
 
-__all__ = ['physical_components', 'circuit_logic', 'optimization_problems']
+__all__ = ['physical_components', 'circuit_logic', 'optimization_problems', 'optimization']

@@ -95,6 +95,22 @@ int qg_mcclean_grad_run(qg_plan* plan, const int32_t* axes, const double* angles
 int qg_mcclean_expec_val(qg_plan* plan, const int32_t* axes, const double* angles, const qg_term* terms,
                          int nterms, const double* ini_state, double* E_out);
 
+/* Projector probabilities of parameter-shifted McClean circuits, for the
+ * shot-based parameter-shift gradients McClean.sample_grad and
+ * McClean.sample_grad_with_component_sampling (mc_clean.py:117-198).  Shift s
+ * adds shift_delta[s] to angles[shift_param[s]] (flat index i*n + q; the
+ * reference's manual rotation of pi/2 or -pi/2 after layer i, which is the
+ * same circuit); prob[s*nterms + k] = |P_k^+ psi_s|^2 of term k in the final
+ * state psi_s of that circuit (weights ignored), as qg_plan_term_prob gives
+ * it.  The unshifted circuit runs first and keeps its layer-end states
+ * (history) or checkpoints; each shifted circuit restarts from the nearest
+ * one before its shifted layer, like the reference's restart from
+ * state_history[i].  Afterwards the plan's state is the unshifted circuit's
+ * final state.  The shots are drawn by the caller.  Needs n <= 31. */
+int qg_mcclean_shift_term_prob(qg_plan* plan, const int32_t* axes, const double* angles, const double* ini_state,
+                               int nshift, const int32_t* shift_param, const double* shift_delta,
+                               const qg_term* terms, int nterms, double* prob);
+
 /* ---- QAOA circuit (qradient/circuit_logic/qaoa.py) -----------------------
  * Plan for `batch` circuits of n qubits and p layers each. */
 int qg_qaoa_plan(qg_ctx* ctx, int n, int p, int dtype, int batch, qg_plan** out);
@@ -136,6 +152,13 @@ int qg_plan_get_state(qg_plan* plan, double* out);
  * state is that of the first circuit of the last chunk on the pass kernels,
  * and absent after the whole-circuit kernel. */
 int qg_plan_term_prob(qg_plan* plan, const qg_term* terms, int nterms, double* prob);
+/* E = sum_k w_k <psi|P_k|psi> over terms[0, nterms) in the state of the plan's
+ * last forward run: ParametrizedCircuit.expec_val (base.py:17-20) without
+ * forming the matrix, for x / y / z / zz terms at any n <= 31.  Used for the
+ * full-observable expectation of McClean.grad_run_with_component_sampling
+ * (mc_clean.py:80-115).  A grad run with checkpoints overwrites the final
+ * state (QG_ERR_INVALID then). */
+int qg_plan_expec_val(qg_plan* plan, const qg_term* terms, int nterms, double* E_out);
 /* Device time of the last run in milliseconds (hipEvents around the launches). */
 int qg_plan_last_ms(const qg_plan* plan, double* ms);
 /* Record a HIP event before every kernel launch of later runs (on = 1). */

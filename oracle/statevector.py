@@ -441,6 +441,104 @@ def mcclean_grad_run_adjoint(n, axes, angles, obs, ini_state=None, entangler='cn
 
 
 # --------------------------------------------------------------------------
+# Sampled McClean gradients (mc_clean.py:80-198)
+# --------------------------------------------------------------------------
+def component_observable(obs, k):
+    """The dict observable of term k alone, with its weight: the reference's
+    ``component_array[k]`` = weight * operator (observable.py:44-73)."""
+    n, terms = observable_terms(obs)
+    kind, i, j, w = terms[k]
+    if kind == 'zz':
+        zz = np.full((n, n), None)
+        zz[i, j] = w
+        return {'zz': zz}
+    arr = np.array([None] * n, dtype=object)
+    arr[i] = w
+    return {kind: arr}
+
+
+def weight_distribution(obs):
+    """Component sampling distribution.  The reference's w / sum(w)
+    (observable.py:75-79) for non-negative weights; |w| / sum|w| in general
+    (the reference's is not a distribution when a weight is negative)."""
+    _, terms = observable_terms(obs)
+    w = np.abs(np.array([t[3] for t in terms]))
+    return w / w.sum()
+
+
+def mcclean_grad_run_component(n, axes, angles, obs, component, ini_state=None, entangler='cnot'):
+    """McClean.grad_run_with_component_sampling (mc_clean.py:80-115) for a
+    given component: the forward sweep with history, E of the whole
+    observable, then lambda = component_array[k] psi_L and the reverse sweep
+    of grad_run (mc_clean.py:67-78)."""
+    E = mcclean_run_expec_val(n, axes, angles, obs, ini_state, entangler)
+    _, grad = mcclean_grad_run(n, axes, angles, component_observable(obs, component), ini_state, entangler)
+    return E, grad
+
+
+def mcclean_shift_states(n, axes, angles, ini_state=None, entangler='cnot'):
+    """The shifted final states of McClean.sample_grad (mc_clean.py:117-155),
+    as the reference forms them: state_history[i] is the state after layer
+    i's rotations (mc_clean.py:126-131); for each (i, dq) the state restarts
+    from state_history[i], gets the manual rotation of +pi/2 about axes[i, dq]
+    on qubit dq (mc_clean.py:394-403) and runs layers i+1..L-1, then the same
+    with -pi/2.  Yields (i, dq, psi_plus, psi_minus)."""
+    L = axes.shape[0]
+    v = mcclean_prepare(n, ini_state)
+    hist = []
+    for i in range(L):
+        entangle(v, entangler)
+        for q in range(n):
+            rot(v, axes[i, q], angles[i, q], q)
+        hist.append(v.copy())
+    for i in range(L):
+        for dq in range(n):
+            out = []
+            for shift in (np.pi / 2, -np.pi / 2):
+                u = hist[i].copy()
+                rot(u, axes[i, dq], shift, dq)
+                for j in range(i + 1, L):
+                    entangle(u, entangler)
+                    for q in range(n):
+                        rot(u, axes[j, q], angles[j, q], q)
+                out.append(u)
+            yield i, dq, out[0], out[1]
+
+
+def sample_component_expec_val(v, obs, shot_num, component, random_state=None):
+    """ParametrizedCircuit.sample_component_expec_val (base.py:36-46)."""
+    from scipy import stats
+    probs, weights = term_probs(v, obs)
+    prob, weight = probs[component], weights[component]
+    rnd_var = stats.rv_discrete(values=([0, 1], [prob, 1 - prob]))
+    return np.array([weight, -weight])[rnd_var.rvs(size=shot_num, random_state=random_state)].mean()
+
+
+def mcclean_sample_grad(n, axes, angles, obs, shot_num, exact_expec_val=True, ini_state=None, entangler='cnot',
+                        component_sampling=False):
+    """McClean.sample_grad (mc_clean.py:117-155) and, with
+    ``component_sampling``, sample_grad_with_component_sampling
+    (mc_clean.py:157-198): random numbers from numpy's global RandomState in
+    the reference's order (E's shots, then per parameter the component draw,
+    o_plus's shots, o_minus's shots)."""
+    L = axes.shape[0]
+    final = mcclean_forward(n, axes, angles, ini_state, entangler)
+    E = expec_val(final, obs) if exact_expec_val else sample_expec_val(final, obs, shot_num)
+    grad = np.empty((L, n))
+    dist = weight_distribution(obs) if component_sampling else None
+    for i, dq, vp, vm in mcclean_shift_states(n, axes, angles, ini_state, entangler):
+        if component_sampling:
+            c = np.random.choice(np.arange(dist.size), p=dist)
+            o_plus = sample_component_expec_val(vp, obs, shot_num, c)
+            o_minus = sample_component_expec_val(vm, obs, shot_num, c)
+        else:
+            o_plus = sample_expec_val(vp, obs, shot_num)
+            o_minus = sample_expec_val(vm, obs, shot_num)
+        grad[i, dq] = .5 * (o_plus - o_minus)
+    return E, grad
+
+
+# --------------------------------------------------------------------------
 # QAOA (qaoa.py:9-70; classical Hamiltonian state.py:261-321)
 # --------------------------------------------------------------------------
 def classical_ham(n, obs):

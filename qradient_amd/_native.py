@@ -46,6 +46,7 @@ SIGNATURES = {
     'qg_mcclean_plan': [_vp, _i, _i, _i, _i, _i, _pp],
     'qg_mcclean_grad_run': [_vp, _ip, _dp, _tp, _i, _dp, _dp, _dp],
     'qg_mcclean_expec_val': [_vp, _ip, _dp, _tp, _i, _dp, _dp],
+    'qg_mcclean_shift_term_prob': [_vp, _ip, _dp, _dp, _i, _ip, _dp, _tp, _i, _dp],
     'qg_qaoa_plan': [_vp, _i, _i, _i, _i, _pp],
     'qg_qaoa_grad_run': [_vp, _i, _tp, _ip, _dp, _dp, _dp, _dp],
     'qg_qaoa_expec_val': [_vp, _i, _tp, _ip, _dp, _dp, _dp],
@@ -80,6 +81,7 @@ SIGNATURES = {
     'qg_state_norm_error': [_vp, _dp],
     'qg_state_term_prob': [_vp, _tp, _i, _dp],
     'qg_plan_term_prob': [_vp, _tp, _i, _dp],
+    'qg_plan_expec_val': [_vp, _tp, _i, _dp],
     'qg_state_scale': [_vp, _d, _d],
 }
 _RESTYPES = {'qg_last_error': ctypes.c_char_p}

@@ -109,6 +109,12 @@ class ParametrizedCircuit:
 
     def expec_val(self):
         """<vec|O|vec> of the current state (base.py:17-20), FP64 on the device."""
+        if self._plan_state:
+            # the last run's final state is still in the plan's workspace
+            arr, k = self.observable.c_terms()
+            E = ctypes.c_double()
+            _native.call('qg_plan_expec_val', self._plan, arr, k, ctypes.byref(E))
+            return np.float64(E.value)
         return self.state.expec_val(self.observable)
 
     def _term_probs(self, terms):

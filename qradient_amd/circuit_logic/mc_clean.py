@@ -19,7 +19,7 @@ import numpy as np
 
 from .. import _native
 from ..physical_components import Gates
-from .base import ParametrizedCircuit
+from .base import ParametrizedCircuit, sample_projectors
 
 
 class McClean(ParametrizedCircuit):
@@ -95,6 +95,108 @@ class McClean(ParametrizedCircuit):
         if not exact_expec_val:
             return self.sample_expec_val(shot_num)
         return np.float64(E.value)
+
+    def grad_run_with_component_sampling(self, hide_progbar=True, ini_state=None):
+        """(E, grad[L, n]) with the adjoint sweep seeded by one sampled
+        observable component (mc_clean.py:80-115).
+
+        E is the whole observable's expectation.  The gradient is that of
+        <psi|w_k P_k|psi> for the term k drawn by
+        ``np.random.choice(num_components, p=weight_distribution)`` from the
+        global RNG, one draw per call as in the reference (which draws after
+        the forward sweep; the sweep takes no random numbers, so the stream is
+        the same).  Needs ``use_observable_components=True``.
+        """
+        obs = self.observable
+        k = np.random.choice(np.arange(obs.num_components), p=obs.weight_distribution)
+        axes, angles = self._params()
+        _keep, ini = self._ini(ini_state, self.qnum)
+        arr, nk = _native.terms_array([obs.terms[k]])
+        E = ctypes.c_double()
+        grad = np.empty((self.lnum, self.qnum), dtype='double')
+        plan = self._get_plan()
+        _native.call('qg_mcclean_grad_run', plan, _native.iptr(axes), _native.dptr(angles), arr, nk, ini,
+                     ctypes.byref(E), _native.dptr(grad))
+        self._plan_state = False
+        full, nf = obs.c_terms()
+        status = _native.load().qg_plan_expec_val(plan, full, nf, ctypes.byref(E))
+        if status == _native.QG_ERR_INVALID:
+            # a checkpointed adjoint sweep uncomputed the final state: run the
+            # forward sweep again for the whole observable
+            _native.call('qg_mcclean_expec_val', plan, _native.iptr(axes), _native.dptr(angles), full, nf, ini,
+                         ctypes.byref(E))
+        else:
+            _native.check(status)
+        return np.float64(E.value), grad
+
+    def shifted_term_probs(self, shift_param, shift_delta, terms, ini_state=None):
+        """Projector probabilities of terms (``observable.terms`` entries) in
+        the final states of the McClean circuits with angles.flat[p] shifted
+        by delta, one row per (p, delta) pair; afterwards ``state`` is the
+        unshifted circuit's final state."""
+        axes, angles = self._params()
+        _keep, ini = self._ini(ini_state, self.qnum)
+        sp_ = np.ascontiguousarray(shift_param, dtype=np.int32)
+        sd = np.ascontiguousarray(shift_delta, dtype=np.float64)
+        if sp_.shape != sd.shape or sp_.ndim != 1:
+            raise ValueError('shift_param and shift_delta must be 1-d arrays of one length')
+        arr, k = _native.terms_array(terms)
+        probs = np.empty((sp_.size, k))
+        _native.call('qg_mcclean_shift_term_prob', self._get_plan(), _native.iptr(axes), _native.dptr(angles), ini,
+                     int(sp_.size), _native.iptr(sp_), _native.dptr(sd), arr, k, _native.dptr(probs))
+        self._plan_state = True
+        return probs
+
+    def _shift_run(self, shot_num, exact_expec_val, ini_state):
+        """Shared part of the parameter-shift methods: the unshifted circuit's
+        E (exact, or its shot estimate), then the probabilities of every term
+        in the +pi/2 and -pi/2 shifted circuit of every parameter, rows in
+        the reference's loop order (layer, qubit, then +, -)."""
+        npar = self.lnum * self.qnum
+        params = np.repeat(np.arange(npar, dtype=np.int32), 2)
+        deltas = np.tile(np.array([np.pi / 2, -np.pi / 2]), npar)
+        terms = self.observable.terms
+        probs = self.shifted_term_probs(params, deltas, terms, ini_state)
+        if exact_expec_val:
+            E = self.expec_val()
+        else:
+            E = self.sample_expec_val(shot_num)
+        return E, probs, np.array([t[3] for t in terms])
+
+    def sample_grad(self, hide_progbar=True, shot_num=1, exact_expec_val=True, ini_state=None):
+        """(E, grad[L, n]) by the parameter-shift rule on finite-shot
+        expectation values (mc_clean.py:117-155): grad[i, q] =
+        (o_plus - o_minus) / 2 with o_+- the shot_num-shot estimates of the
+        circuit with angles[i, q] shifted by +-pi/2.  The shifted circuits run
+        on the device and give the projector probabilities of every term; the
+        shots are drawn on the host from the global RNG in the reference's
+        order (E's shots first when exact_expec_val is False, then per
+        parameter o_plus's, then o_minus's), so a seeded call gives the
+        reference's estimate."""
+        E, probs, w = self._shift_run(shot_num, exact_expec_val, ini_state)
+        grad = np.empty((self.lnum, self.qnum), dtype='double')
+        for k in range(self.lnum * self.qnum):
+            o_plus = sample_projectors(probs[2 * k], w, shot_num)
+            o_minus = sample_projectors(probs[2 * k + 1], w, shot_num)
+            grad.flat[k] = .5 * (o_plus - o_minus)
+        return E, grad
+
+    def sample_grad_with_component_sampling(self, hide_progbar=True, shot_num=1, exact_expec_val=True,
+                                            ini_state=None):
+        """As ``sample_grad``, but each parameter's two estimates measure one
+        observable component, drawn per parameter with
+        ``np.random.choice(num_components, p=weight_distribution)`` before its
+        shots (mc_clean.py:157-198).  Needs ``use_observable_components=True``."""
+        obs = self.observable
+        ncomp, dist = obs.num_components, obs.weight_distribution
+        E, probs, w = self._shift_run(shot_num, exact_expec_val, ini_state)
+        grad = np.empty((self.lnum, self.qnum), dtype='double')
+        for k in range(self.lnum * self.qnum):
+            c = np.random.choice(np.arange(ncomp), p=dist)
+            o_plus = sample_projectors(probs[2 * k, c:c + 1], w[c:c + 1], shot_num)
+            o_minus = sample_projectors(probs[2 * k + 1, c:c + 1], w[c:c + 1], shot_num)
+            grad.flat[k] = .5 * (o_plus - o_minus)
+        return E, grad
 
     def final_state(self):
         """The state after the last forward run (complex128 host copy)."""

@@ -7,6 +7,7 @@ dE/d gamma_i.  Unlike the reference, an ``ini_state`` array is copied, never
 mutated.
 """
 import ctypes
+import warnings
 
 import numpy as np
 
@@ -69,6 +70,11 @@ class Qaoa(ParametrizedCircuit):
                          _native.dptr(gammas), ini, _native.dptr(E), _native.dptr(grad))
         self._plan_state = False
         return np.float64(E[0]), grad
+
+    def sample_grad(self, betas, gammas, shot_num=1, hide_progbar=True, exact_expec_val=True, ini_state=None):
+        """Not implemented in the reference either (qaoa.py:72-81): warns and
+        returns None."""
+        warnings.warn('Not implemented yet.')
 
     def run_expec_val(self, betas, gammas, hide_progbar=True, exact_expec_val=True, shot_num=1, ini_state=None):
         """Forward circuit, then the exact expectation or, with

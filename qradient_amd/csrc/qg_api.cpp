@@ -508,11 +508,12 @@ extern "C" int qg_plan_last_ms(const qg_plan* p, double* ms) {
 // (weights ignored): the probability of the +1 outcome that the projectors of
 // observable.py:126-177 give in base.py:22-46, also for a state whose norm is
 // not exactly 1.  One pass over v per kTermsPerLaunch terms.
-static int term_prob(qg_ctx* c, int dtype, const void* v, int n, const qg_term* terms, int nterms, double* prob) {
-  if (nterms < 0 || (nterms > 0 && (!terms || !prob))) return fail(QG_ERR_INVALID, "NULL argument");
-  if (n > 31) return fail(QG_ERR_UNSUPPORTED, "term probabilities need n <= 31");
-  if (nterms == 0) return QG_OK;
-  std::vector<PTerm> h(nterms + 1);
+// The term table of the term-expectation kernel: terms[0, nterms) in order,
+// then the identity (|v|^2).
+static int pterm_table(int n, const qg_term* terms, int nterms, std::vector<PTerm>& h) {
+  if (nterms < 0 || (nterms > 0 && !terms)) return fail(QG_ERR_INVALID, "NULL argument");
+  if (n > 31) return fail(QG_ERR_UNSUPPORTED, "term expectations need n <= 31");
+  h.assign(nterms + 1, PTerm{});
   for (int k = 0; k < nterms; ++k) {
     const qg_term& t = terms[k];
     if (t.i < 0 || t.i >= n) return fail(QG_ERR_INVALID, "observable: qubit index out of range");
@@ -534,6 +535,14 @@ static int term_prob(qg_ctx* c, int dtype, const void* v, int n, const qg_term* 
     }
   }
   h[nterms] = PTerm{3, 0, -1, 0};  // |v|^2
+  return QG_OK;
+}
+
+// e[k] = <v|P_k|v> for the terms in order, e[nterms] = |v|^2 (weights
+// ignored).  One pass over v per kTermsPerLaunch terms.
+static int term_expec(qg_ctx* c, int dtype, const void* v, int n, const qg_term* terms, int nterms, double* e) {
+  std::vector<PTerm> h;
+  CHK(pterm_table(n, terms, nterms, h));
   const int nt = nterms + 1;
   HIPCHK(hipSetDevice(c->device));
   const size_t tb = ((sizeof(PTerm) * nt + 255) / 256) * 256;
@@ -543,14 +552,38 @@ static int term_prob(qg_ctx* c, int dtype, const void* v, int n, const qg_term* 
   PTerm* d_t = (PTerm*)buf;
   double* d_part = (double*)(buf + tb);
   double* d_out = (double*)(buf + tb + pb);
-  std::vector<double> e(nt);
   hipError_t err = hipMemcpyAsync(d_t, h.data(), sizeof(PTerm) * nt, hipMemcpyHostToDevice, c->stream);
   if (err == hipSuccess) err = simple_term_expec(dtype, v, n, d_t, nt, d_part, d_out, c->stream);
-  if (err == hipSuccess) err = hipMemcpyAsync(e.data(), d_out, sizeof(double) * nt, hipMemcpyDeviceToHost, c->stream);
+  if (err == hipSuccess) err = hipMemcpyAsync(e, d_out, sizeof(double) * nt, hipMemcpyDeviceToHost, c->stream);
   if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
   (void)hipFree(buf);
-  if (err != hipSuccess) return fail(QG_ERR_HIP, std::string("term probabilities: ") + hipGetErrorString(err));
+  if (err != hipSuccess) return fail(QG_ERR_HIP, std::string("term expectations: ") + hipGetErrorString(err));
+  return QG_OK;
+}
+
+// prob[k] = <v|(I + P_k)/2|v> = |P_k^+ v|^2 for each term, in the given order
+// (weights ignored): the probability of the +1 outcome that the projectors of
+// observable.py:126-177 give in base.py:22-46, also for a state whose norm is
+// not exactly 1.
+static int term_prob(qg_ctx* c, int dtype, const void* v, int n, const qg_term* terms, int nterms, double* prob) {
+  if (nterms < 0 || (nterms > 0 && (!terms || !prob))) return fail(QG_ERR_INVALID, "NULL argument");
+  if (nterms == 0) return QG_OK;
+  std::vector<double> e(nterms + 1);
+  CHK(term_expec(c, dtype, v, n, terms, nterms, e.data()));
   for (int k = 0; k < nterms; ++k) prob[k] = 0.5 * (e[nterms] + e[k]);
+  return QG_OK;
+}
+
+extern "C" int qg_plan_expec_val(qg_plan* p, const qg_term* terms, int nterms, double* E_out) {
+  if (!p || !E_out) return fail(QG_ERR_INVALID, "NULL argument");
+  if (!p->last_state_ptr) return fail(QG_ERR_INVALID, "no state: run the circuit first");
+  for (int k = 0; k < nterms; ++k)
+    if (terms && !std::isfinite(terms[k].w)) return fail(QG_ERR_INVALID, "observable: non-finite weight");
+  std::vector<double> e(nterms + 1);
+  CHK(term_expec(p->ctx, p->dtype, p->last_state_ptr, p->n, terms, nterms, e.data()));
+  double E = 0.0;
+  for (int k = 0; k < nterms; ++k) E += terms[k].w * e[k];
+  *E_out = E;
   return QG_OK;
 }
 
@@ -1356,6 +1389,8 @@ static int mcclean_run(qg_plan* p, const int32_t* axes, const double* angles, co
   }
   int nd = 0;
   CHK(mcclean_schedule(p, axes, angles, obs, ini_state != nullptr, grad, nd));
+  // uncompute passes of a checkpointed adjoint overwrite the final state
+  if (grad && !p->history) p->last_state_ptr = nullptr;
   // rows written by each pass and the finalize table
   const int P = (int)p->geo.size();
   std::vector<std::vector<int>> out_rows(grad ? 1 + L * n : 1);
@@ -1406,6 +1441,142 @@ extern "C" int qg_mcclean_grad_run(qg_plan* p, const int32_t* axes, const double
 extern "C" int qg_mcclean_expec_val(qg_plan* p, const int32_t* axes, const double* angles, const qg_term* terms,
                                     int nterms, const double* ini_state, double* E_out) {
   return mcclean_run(p, axes, angles, terms, nterms, ini_state, E_out, nullptr, false);
+}
+
+// ---------------------------------------------------------------------------
+// Parameter-shifted McClean circuits (McClean.sample_grad, mc_clean.py:117-198)
+// ---------------------------------------------------------------------------
+
+// Appends the forward passes of layers [first, L) to the plan's launch list,
+// from the state at src (nullptr: the RY(pi/4) product state, generated by the
+// first pass).  The states go to the two buffers buf: the low pass gathers
+// out of place, high passes work in place, so src is only read.  Returns the
+// buffer that holds the final state.
+static void* mcclean_forward_suffix(qg_plan* p, const int32_t* axes, const double* angles, int first,
+                                    const void* src, void* const buf[2], int& nd) {
+  const int n = p->n, L = p->L, P = (int)p->geo.size();
+  const uint32_t ent = p->ent == QG_ENT_CNOT ? F_GATHER : F_CZ_PRE;
+  const void* cur = src;
+  for (int i = first; i < L; ++i) {
+    for (int g = 0; g < P; ++g) {
+      const Geo& G0 = p->geo[g];
+      bool tf = false;
+      const Geo G = with_role(G0, p->dtype, ROLE_FWD, &tf);
+      Launch l;
+      l.desc = nd;
+      l.R = G.R;
+      l.K = G.K;
+      l.post = tf;
+      l.threads = G.threads;
+      l.grid = grid_for_pass(p, G, false, tf, 0);
+      l.phase = 0;
+      l.two = false;
+      l.kind = 0;
+      p->launches.push_back(l);
+      PassDesc& d = p->h_desc[nd++];
+      desc_geo(d, n, G);
+      d.grid = p->gmax;
+      d.map = p->d_map;
+      d.init_tab = p->d_init;
+      d.dterms = p->d_terms;
+      d.ndterms = 0;
+      d.part = p->d_part;
+      uint32_t f = F_ROT_A | F_STORE_A;
+      f |= (g == 0) ? (ent | (cur ? F_LOAD_A : F_INIT_TAB)) : F_LOAD_A;
+      void* out = (g == 0) ? (cur == buf[0] ? buf[1] : buf[0]) : (void*)cur;
+      d.flags = f;
+      d.a_in = cur;
+      d.a_out = out;
+      for (int lb = G0.own_lo; lb < G0.own_hi; ++lb) {
+        const int b = (lb < G0.c) ? lb : G0.p + (lb - G0.c);
+        const int q = n - 1 - b;
+        set_gate(d, lb, axes[i * n + q], angles[i * n + q], 1.0);
+      }
+      cur = out;
+    }
+  }
+  return (void*)cur;
+}
+
+extern "C" int qg_mcclean_shift_term_prob(qg_plan* p, const int32_t* axes, const double* angles,
+                                          const double* ini_state, int nshift, const int32_t* shift_param,
+                                          const double* shift_delta, const qg_term* terms, int nterms,
+                                          double* prob) {
+  if (!p || p->kind != PK_MCCLEAN) return fail(QG_ERR_INVALID, "not a McClean plan");
+  if (nshift < 0 || nterms < 0) return fail(QG_ERR_INVALID, "negative count");
+  if (nshift > 0 && (!shift_param || !shift_delta || (nterms > 0 && !prob)))
+    return fail(QG_ERR_INVALID, "NULL argument");
+  const int n = p->n, L = p->L;
+  for (int s = 0; s < nshift; ++s) {
+    if (shift_param[s] < 0 || shift_param[s] >= L * n)
+      return fail(QG_ERR_INVALID, "shift parameter index out of range");
+    if (!std::isfinite(shift_delta[s])) return fail(QG_ERR_INVALID, "non-finite shift");
+  }
+  std::vector<PTerm> h;
+  CHK(pterm_table(n, terms, nterms, h));
+  // the unshifted circuit through the pass kernels, keeping its layer-end
+  // states (history) or checkpoints: every shifted circuit starts from the
+  // nearest one before its shifted layer
+  const bool fused = p->fused;
+  p->fused = false;
+  double E = 0.0;
+  const qg_term none{QG_TERM_Z, 0, -1, 0, 0.0};  // the forward run's own observable: unused
+  const int r = mcclean_run(p, axes, angles, &none, 1, ini_state, &E, nullptr, false);
+  p->fused = fused;
+  CHK(r);
+  if (nshift == 0 || nterms == 0) return QG_OK;
+  qg_ctx* c = p->ctx;
+  const int P = (int)p->geo.size();
+  std::vector<const void*> layer_end(L, nullptr);
+  if (p->history) {
+    for (int i = 0; i < L; ++i) layer_end[i] = wsp(p, p->off_hist + ((size_t)i * P + P - 1) * p->S);
+  } else {
+    int m = 0;
+    for (int i = 0; i < L; ++i)
+      if ((i + 1) % p->ckpt == 0 || i == L - 1) layer_end[i] = wsp(p, p->off_ckpt + (size_t)(m++) * p->S);
+  }
+  const void* start0 = ini_state ? wsp(p, p->off_prep) : nullptr;
+  // the adjoint's two lambda buffers are free in a forward-only run
+  void* buf[2] = {wsp(p, p->off_lam), wsp(p, p->off_lam + p->S)};
+  const int nt = nterms + 1;
+  const size_t tb = ((sizeof(PTerm) * nt + 255) / 256) * 256;
+  const size_t pb = sizeof(double) * kTermBlocks * kTermsPerLaunch;
+  char* dbuf = nullptr;
+  HIPCHK(hipMalloc(&dbuf, tb + pb + sizeof(double) * (size_t)nt * nshift));
+  PTerm* d_t = (PTerm*)dbuf;
+  double* d_part = (double*)(dbuf + tb);
+  double* d_res = (double*)(dbuf + tb + pb);
+  std::vector<double> res((size_t)nt * nshift);
+  std::vector<double> ang(angles, angles + (size_t)L * n);
+  int rc = QG_OK;
+  hipError_t e = hipMemcpyAsync(d_t, h.data(), sizeof(PTerm) * nt, hipMemcpyHostToDevice, c->stream);
+  for (int s = 0; s < nshift && e == hipSuccess && rc == QG_OK; ++s) {
+    const int k = shift_param[s], i = k / n;
+    int j = i - 1;
+    while (j >= 0 && !layer_end[j]) --j;
+    ang[k] = angles[k] + shift_delta[s];
+    p->launches.clear();
+    int nd = 0;
+    void* fin = mcclean_forward_suffix(p, axes, ang.data(), j + 1, j >= 0 ? layer_end[j] : start0, buf, nd);
+    ang[k] = angles[k];
+    rc = run_launches(p, nd);
+    if (rc != QG_OK) break;
+    e = simple_term_expec(p->dtype, fin, n, d_t, nt, d_part, d_res + (size_t)s * nt, c->stream);
+    // the next shift rewrites the pinned descriptor table this one uploads from
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  }
+  if (e == hipSuccess && rc == QG_OK)
+    e = hipMemcpyAsync(res.data(), d_res, sizeof(double) * res.size(), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess && rc == QG_OK) e = hipStreamSynchronize(c->stream);
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(dbuf);
+  // the plan's state stays the unshifted circuit's final state
+  if (rc != QG_OK) return rc;
+  if (e != hipSuccess) return fail(QG_ERR_HIP, std::string("shifted circuits: ") + hipGetErrorString(e));
+  for (int s = 0; s < nshift; ++s)
+    for (int t = 0; t < nterms; ++t)
+      prob[(size_t)s * nterms + t] = 0.5 * (res[(size_t)s * nt + nterms] + res[(size_t)s * nt + t]);
+  return QG_OK;
 }
 
 // ---------------------------------------------------------------------------

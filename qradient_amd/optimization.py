@@ -13,9 +13,13 @@ host.
 (``QaoaBatch``) per step and the same update rule applied to the [B, p, 2]
 parameter block, each circuit with its own state of the rule.
 
-Shot-based gradients (``shot_num > 0``: sample_grad*, mc_clean.py:117-198,
-qaoa.py:83-158) and component-sampled gradients are not on the GPU path and
-raise NotImplementedError.
+``McCleanOpt.step`` dispatches like the reference (optimization.py:69-91):
+``shot_num == 0`` takes grad_run or, with ``component_sampling``,
+grad_run_with_component_sampling; ``shot_num > 0`` with ``dense_mode=False``
+takes the shot-based parameter-shift gradients sample_grad /
+sample_grad_with_component_sampling.  The dense-eigenbasis variants
+(``dense_mode=True`` with shots: sample_grad_dense*, O(4^n) in the
+reference) raise NotImplementedError, as does QaoaOpt with shots.
 """
 import numpy as np
 
@@ -51,9 +55,9 @@ class ParametrizedCircuitOptimizer:
             print('Maximum amount of iterations reached: {}.'.format(self.max_iter))
 
 
-def _no_shots(shot_num, component_sampling=False):
-    if shot_num != 0 or component_sampling:
-        raise NotImplementedError('shot-based and component-sampled gradients are not on the GPU grad_run path')
+def _no_dense(what):
+    raise NotImplementedError('{}: the dense-eigenbasis sampled gradients are not built; use '
+                              'dense_mode=False (McClean) or shot_num=0'.format(what))
 
 
 class McCleanOpt(ParametrizedCircuitOptimizer):
@@ -75,8 +79,17 @@ class McCleanOpt(ParametrizedCircuitOptimizer):
 
     def step(self, shot_num=0, dense_mode=True, component_sampling=False):
         self._check_iter()
-        _no_shots(shot_num, component_sampling)
-        e, g = self.circuit.grad_run()
+        if shot_num == 0:
+            if component_sampling:
+                e, g = self.circuit.grad_run_with_component_sampling()
+            else:
+                e, g = self.circuit.grad_run()
+        elif dense_mode:
+            _no_dense('sample_grad_dense')
+        elif component_sampling:
+            e, g = self.circuit.sample_grad_with_component_sampling(shot_num=shot_num)
+        else:
+            e, g = self.circuit.sample_grad(shot_num=shot_num)
         self.cost_history[self.iter] = e
         self.optimizer.step(g, e)
         self.iter += 1
@@ -106,9 +119,10 @@ class QaoaOpt(ParametrizedCircuitOptimizer):
 
     def step(self, shot_num=0, dense_mode=True):
         self._check_iter()
-        if shot_num != 0 and not dense_mode:
-            raise ValueError('dense_mode must be True, sparse sampling method is not implmented yet.')
-        _no_shots(shot_num)
+        if shot_num != 0:
+            if not dense_mode:
+                raise ValueError('dense_mode must be True, sparse sampling method is not implmented yet.')
+            _no_dense('Qaoa.sample_grad_dense')
         e, g = self.circuit.grad_run(self.param_history[self.iter, :, 0], self.param_history[self.iter, :, 1])
         self.cost_history[self.iter] = e
         self.optimizer.step(g, e)

@@ -124,12 +124,44 @@ def test_unknown_optimizer_and_history_length():
         opt.step()
 
 
-def test_shot_gradients_are_not_on_the_gpu_path():
+class TracingMcClean(QuadMcClean):
+    """Records which gradient method a step called (optimization.py:69-84)."""
+
+    def __init__(self, *a, **k):
+        super().__init__(*a, **k)
+        self.calls = []
+
+    def _traced(name):
+        def method(self, **kw):
+            self.calls.append((name, kw))
+            return self.grad_run()
+        return method
+
+    grad_run_with_component_sampling = _traced('grad_run_with_component_sampling')
+    sample_grad = _traced('sample_grad')
+    sample_grad_with_component_sampling = _traced('sample_grad_with_component_sampling')
+
+
+def test_step_dispatches_like_the_reference():
+    circ = TracingMcClean(1, 2)
+    opt = McCleanOpt(circ, {'name': 'Adam'}, max_iter=10)
+    opt.step(component_sampling=True)
+    opt.step(shot_num=10, dense_mode=False)
+    opt.step(shot_num=5, dense_mode=False, component_sampling=True)
+    assert circ.calls == [('grad_run_with_component_sampling', {}), ('sample_grad', {'shot_num': 10}),
+                          ('sample_grad_with_component_sampling', {'shot_num': 5})]
+    assert opt.iter == 3
+
+
+def test_dense_shot_gradients_are_not_built():
     opt = McCleanOpt(QuadMcClean(1, 2), {'name': 'Adam'}, max_iter=3)
     with pytest.raises(NotImplementedError):
         opt.step(shot_num=10)
+    qopt = QaoaOpt(QuadQaoa(2), {'name': 'Adam'}, np.zeros(2), np.zeros(2), max_iter=3)
     with pytest.raises(NotImplementedError):
-        opt.step(component_sampling=True)
+        qopt.step(shot_num=10)
+    with pytest.raises(ValueError):
+        qopt.step(shot_num=10, dense_mode=False)
 
 
 def test_mcclean_reset():

@@ -111,6 +111,20 @@ int qg_mcclean_shift_term_prob(qg_plan* plan, const int32_t* axes, const double*
                                int nshift, const int32_t* shift_param, const double* shift_delta,
                                const qg_term* terms, int nterms, double* prob);
 
+/* The same shifted circuits, measured in a basis ordered by `order` (int32
+ * [2^n], order[k] = the amplitude at position k; NULL: natural order): for
+ * shift s and shot m, idx[s*shots + m] = the first position k whose
+ * cumulative probability sum_{k' <= k} |psi_s[order[k']]|^2 reaches
+ * u[s*shots + m], or 0 when none does.  That is scipy's
+ * rv_discrete(values=(arange(2^n), dist)).rvs draw for the uniform u, as
+ * McClean.sample_grad_dense samples the observable's eigenbasis
+ * (mc_clean.py:207-268): for a diagonal observable, order sorts its diagonal
+ * ascending (the order of np.linalg.eigh's eigenvalues) and the drawn
+ * eigenvalue is diag[order[idx]].  The uniforms are drawn by the caller. */
+int qg_mcclean_shift_sample(qg_plan* plan, const int32_t* axes, const double* angles, const double* ini_state,
+                            int nshift, const int32_t* shift_param, const double* shift_delta, const int32_t* order,
+                            int shots, const double* u, int32_t* idx);
+
 /* ---- QAOA circuit (qradient/circuit_logic/qaoa.py) -----------------------
  * Plan for `batch` circuits of n qubits and p layers each. */
 int qg_qaoa_plan(qg_ctx* ctx, int n, int p, int dtype, int batch, qg_plan** out);
@@ -226,6 +240,15 @@ int qg_state_norm_error(qg_state* st, double* out);
 /* prob[k] = |P_k^+ vec|^2 for each term (the projector probabilities of
  * base.py:22-46), as qg_plan_term_prob. */
 int qg_state_term_prob(qg_state* st, const qg_term* terms, int nterms, double* prob);
+/* Basis draws from the state, as qg_mcclean_shift_sample draws them from a
+ * shifted circuit's final state: the inverse CDF of |v|^2 over the positions
+ * of `order` (NULL: natural order) at the given uniforms (Qaoa.__sample,
+ * qaoa.py:196-198, with the classical Hamiltonian's values in natural order
+ * as eigenvalues).  Needs n <= 31. */
+int qg_state_sample_basis(qg_state* st, const int32_t* order, int shots, const double* u, int32_t* idx);
+/* dst.vec = src.vec on the device (same n and dtype): the reference's
+ * state.vec[:] = state_history[k] (qaoa.py:139, 151) without a host copy. */
+int qg_state_copy(qg_state* dst, const qg_state* src);
 
 #ifdef __cplusplus
 }

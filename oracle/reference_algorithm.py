@@ -235,3 +235,161 @@ class QaoaRef:
             u = v * (-1.j * self.C)
             grad[i, 1] = -2. * self.hist[2 * i].conj().dot(u).real
         return E, grad
+
+
+# --------------------------------------------------------------------------
+# Dense sampled gradients in the reference's formulation: eigenbasis from a
+# dense eigh, suffix circuits as left-hand-side matrices, rv_discrete draws
+# (mc_clean.py:207-268, 277-350; qaoa.py:83-158, 196-198).  Tiny n only.
+# --------------------------------------------------------------------------
+def _rv_mean(dist, values, shot_num, exact_means):
+    """values[rv_discrete(values=(arange(N), dist)).rvs(size=shot_num)].mean(),
+    or with ``exact_means`` the distribution's mean sum(dist * values) (no
+    random numbers): the deterministic form used to pin the shifted states."""
+    if exact_means:
+        return float(np.dot(dist, values))
+    from scipy import stats
+    rnd_var = stats.rv_discrete(values=(np.arange(dist.size), dist))
+    return values[rnd_var.rvs(size=shot_num)].mean()
+
+
+def _rot_matrix(g, axis, angle, q):
+    """The LeftHandSide.rot factor (mc_clean.py:361-376)."""
+    ident = sp.identity(2 ** g.n, dtype='complex', format='csr')
+    if axis == 0:
+        return np.sin(.5 * angle) * g.xrot_op(q) + np.cos(.5 * angle) * ident
+    if axis == 1:
+        return np.sin(.5 * angle) * g.yrot_op(q) + np.cos(.5 * angle) * ident
+    pos, neg = g.zmasks(q)
+    return np.exp(-.5j * angle) * sp.diags(pos) + np.exp(.5j * angle) * sp.diags(neg)
+
+
+def _component_matrices(obs):
+    """component_array of observable.py:41-73: weight * operator, in order."""
+    n = next(iter(obs.values())).shape[0]
+    out = []
+    for key, P in (('x', _X), ('y', _Y), ('z', _Z)):
+        if key in obs:
+            for i, w in enumerate(obs[key]):
+                if w is not None:
+                    out.append(w * sp.kron(sp.kron(sp.identity(2 ** i), P), sp.identity(2 ** (n - i - 1))))
+    if 'zz' in obs:
+        zz = obs['zz']
+        for i in range(n):
+            for j in range(i + 1, n):
+                if zz[i, j] is not None:
+                    op = sp.kron(sp.kron(sp.identity(2 ** i), _Z), sp.identity(2 ** (j - i - 1)))
+                    op = sp.kron(sp.kron(op, _Z), sp.identity(2 ** (n - j - 1)))
+                    out.append(zz[i, j] * op)
+    return out
+
+
+def mcclean_sample_grad_dense(n, obs, axes, angles, shot_num, exact_expec_val=True, component_sampling=False,
+                              exact_means=False):
+    """McClean.sample_grad_dense (mc_clean.py:207-268) and, with
+    ``component_sampling``, sample_grad_dense_with_component_sampling
+    (mc_clean.py:277-350), in the reference's dense formulation, with numpy's
+    global RandomState in the reference's order."""
+    from . import statevector as sv
+    axes, angles = np.asarray(axes), np.asarray(angles)
+    L = axes.shape[0]
+    ref = McCleanRef(n, obs, axes, angles)
+    g = ref.g
+    if component_sampling:
+        comps = _component_matrices(obs)
+        c = np.random.choice(np.arange(len(comps)), p=sv.weight_distribution(obs))
+        M = comps[c]
+    else:
+        M = ref.O
+    eigenvalues, eigenvectors = np.linalg.eigh(M.toarray())
+    lhs = sp.csr_matrix(eigenvectors.transpose().conj())
+    lhs_history = [lhs.toarray()]
+    l0, _ = g.ladder()
+    for i in range(L - 1):
+        i_inv = L - i - 1
+        for q in range(n):
+            lhs = lhs.dot(_rot_matrix(g, axes[i_inv, q], angles[i_inv, q], q))
+        lhs = lhs.dot(l0)
+        lhs_history.append(lhs.toarray())
+    # forward sweep, history after each layer's rotations (mc_clean.py:226-233)
+    v = np.zeros(2 ** n, dtype='complex')
+    v[0] = 1.
+    for q in range(n):
+        v = _rot_matrix(g, 1, np.pi / 4, q).dot(v)
+    hist = []
+    for i in range(L):
+        v = l0.dot(v)
+        for q in range(n):
+            v = _rot_matrix(g, axes[i, q], angles[i, q], q).dot(v)
+        hist.append(v)
+    E = float(v.conj().dot(ref.O.dot(v)).real) if exact_expec_val else sv.sample_expec_val(v, obs, shot_num)
+    grad = np.empty((L, n))
+    for i in range(L):
+        for q in range(n):
+            s = _rot_matrix(g, axes[i, q], np.pi / 2, q).dot(hist[i])
+            d1 = np.abs(lhs_history[L - i - 1].dot(s)) ** 2
+            s1 = _rv_mean(d1, eigenvalues, shot_num, exact_means)
+            s = _rot_matrix(g, axes[i, q], -np.pi, q).dot(s)
+            d2 = np.abs(lhs_history[L - i - 1].dot(s)) ** 2
+            s2 = _rv_mean(d2, eigenvalues, shot_num, exact_means)
+            grad[i, q] = (s1 - s2) / 2.
+    return E, grad
+
+
+def qaoa_sample_grad_dense(n, obs, betas, gammas, shot_num, exact_expec_val=True, ini_state=None, exact_means=False):
+    """Qaoa.sample_grad_dense (qaoa.py:83-158) in the reference's dense
+    formulation: eigenvalues = the classical Hamiltonian in natural order
+    (qaoa.py:94), suffix circuits as left-hand-side matrices
+    (qaoa.py:127-133), rv_discrete draws (qaoa.py:196-198)."""
+    from . import statevector as sv
+    p = len(betas)
+    C = sv.classical_ham(n, obs)
+    _, terms = sv.observable_terms(obs)
+    comps = [sv.diagonal(n, [t]) for t in terms if t[0] in ('z', 'zz')]
+    ident = sp.identity(2 ** n, dtype='complex', format='csr')
+    x_ops = [-1.j * _one_qubit(n, q, _X) for q in range(n)]
+
+    def xrot_all(m_or_v, b, right):
+        for op in x_ops:
+            f = np.sin(.5 * b) * op + np.cos(.5 * b) * ident
+            m_or_v = m_or_v.dot(f) if right else f.dot(m_or_v)
+        return m_or_v
+
+    lhs = ident
+    lhs_history = [None] * (2 * p)
+    for i in range(p):
+        i_inv = p - i - 1
+        lhs_history[2 * i] = lhs.toarray()
+        lhs = xrot_all(lhs, betas[i_inv], True)
+        lhs_history[2 * i + 1] = lhs.toarray()
+        lhs = lhs.dot(sp.diags(np.exp(-1.j * gammas[i_inv] * C), format='csr'))
+    v = np.full(2 ** n, 2. ** (-.5 * n), dtype='complex') if ini_state is None else \
+        np.array(ini_state, dtype='complex')
+    hist = []
+    for i in range(p):
+        v = v * np.exp(-1.j * gammas[i] * C)
+        hist.append(v)
+        v = xrot_all(v, betas[i], False)
+        hist.append(v)
+    E = float(v.conj().dot(C * v).real) if exact_expec_val else sv.sample_expec_val(v, obs, shot_num)
+    grad = np.empty((p, 2))
+    for i in range(p):
+        total = 0.
+        for comp in comps:
+            s = hist[2 * i] * np.exp(-1.j * (np.pi / 4) * comp)
+            plus = _rv_mean(np.abs(lhs_history[2 * (p - 1 - i) + 1].dot(s)) ** 2, C, shot_num, exact_means)
+            s = s * np.exp(-1.j * (-np.pi / 2) * comp)
+            minus = _rv_mean(np.abs(lhs_history[2 * (p - 1 - i) + 1].dot(s)) ** 2, C, shot_num, exact_means)
+            total += plus - minus
+        grad[i, 1] = total
+        total = 0.
+        for q in range(n):
+            f = np.sin(.25 * np.pi) * x_ops[q] + np.cos(.25 * np.pi) * ident
+            s = f.dot(hist[2 * i + 1])
+            plus = _rv_mean(np.abs(lhs_history[2 * (p - 1 - i)].dot(s)) ** 2, C, shot_num, exact_means)
+            f = np.sin(-.5 * np.pi) * x_ops[q] + np.cos(-.5 * np.pi) * ident
+            s = f.dot(s)
+            minus = _rv_mean(np.abs(lhs_history[2 * (p - 1 - i)].dot(s)) ** 2, C, shot_num, exact_means)
+            total += plus - minus
+        grad[i, 0] = .5 * total
+    return E, grad

@@ -47,6 +47,9 @@ SIGNATURES = {
     'qg_mcclean_grad_run': [_vp, _ip, _dp, _tp, _i, _dp, _dp, _dp],
     'qg_mcclean_expec_val': [_vp, _ip, _dp, _tp, _i, _dp, _dp],
     'qg_mcclean_shift_term_prob': [_vp, _ip, _dp, _dp, _i, _ip, _dp, _tp, _i, _dp],
+    'qg_mcclean_shift_sample': [_vp, _ip, _dp, _dp, _i, _ip, _dp, _ip, _i, _dp, _ip],
+    'qg_state_sample_basis': [_vp, _ip, _i, _dp, _ip],
+    'qg_state_copy': [_vp, _vp],
     'qg_qaoa_plan': [_vp, _i, _i, _i, _i, _pp],
     'qg_qaoa_grad_run': [_vp, _i, _tp, _ip, _dp, _dp, _dp, _dp],
     'qg_qaoa_expec_val': [_vp, _i, _tp, _ip, _dp, _dp, _dp],

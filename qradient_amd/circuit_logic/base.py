@@ -21,7 +21,14 @@ def progbar_range(hide_progbar):
         return np.arange
 
 
-def sample_projectors(probs, weights, shot_num, random_state=None):
+def draw_uniforms(count, shot_num, random_state=None):
+    """``count`` arrays of shot_num uniforms, drawn as ``count`` successive
+    rv_discrete.rvs calls draw them (one ``uniform(size=shot_num)`` each)."""
+    rs = np.random.mtrand._rand if random_state is None else random_state
+    return [rs.uniform(size=int(shot_num)) for _ in range(count)]
+
+
+def sample_projectors(probs, weights, shot_num, random_state=None, uniforms=None):
     """Finite-shot estimate of sum_k w_k <P_k> (base.py:22-34).
 
     For each term in order: shot_num draws of the two-point distribution
@@ -38,10 +45,10 @@ def sample_projectors(probs, weights, shot_num, random_state=None):
     if shot_num < 1:
         raise ValueError('shot_num must be at least 1')
     total = 0.
-    for prob, w in zip(probs, weights):
+    for k, (prob, w) in enumerate(zip(probs, weights)):
         prob = min(max(float(prob), 0.0), 1.0)
         qvals = np.cumsum([prob, 1. - prob])
-        u = rs.uniform(size=shot_num)
+        u = rs.uniform(size=shot_num) if uniforms is None else uniforms[k]
         idx = np.where(u <= qvals[0], 0, np.where(u <= qvals[1], 1, 0))
         total += np.array([w, -w])[idx].mean()
     return total

@@ -14,12 +14,13 @@ cached); ``graph=True`` (replay the pass launches as a HIP graph; for callers
 that make no concurrent device-synchronising HIP calls from other threads).
 """
 import ctypes
+import warnings
 
 import numpy as np
 
 from .. import _native
 from ..physical_components import Gates
-from .base import ParametrizedCircuit, sample_projectors
+from .base import ParametrizedCircuit, draw_uniforms, sample_projectors
 
 
 class McClean(ParametrizedCircuit):
@@ -197,6 +198,97 @@ class McClean(ParametrizedCircuit):
             o_minus = sample_projectors(probs[2 * k + 1, c:c + 1], w[c:c + 1], shot_num)
             grad.flat[k] = .5 * (o_plus - o_minus)
         return E, grad
+
+    def _diagonal_values(self):
+        """D(x) of a diagonal observable, summed in term order as the
+        reference's observable matrix sums its terms (observable.py:44-73);
+        np.linalg.eigh of that matrix returns np.sort(D) exactly."""
+        n = self.qnum
+        x = np.arange(2 ** n, dtype=np.int64)
+        D = np.zeros(2 ** n)
+        for kind, i, j, w in self.observable.terms:
+            s = 1.0 - 2.0 * ((x >> (n - 1 - i)) & 1)
+            if kind == _native.TERM_ZZ:
+                s = s * (1.0 - 2.0 * ((x >> (n - 1 - j)) & 1))
+            D = D + w * s
+        return D
+
+    def sample_grad_dense(self, shot_num=1, hide_progbar=True, exact_expec_val=True, ini_state=None):
+        """(E, grad[L, n]) by the parameter-shift rule with each shifted
+        circuit measured shot_num times in the observable's eigenbasis
+        (mc_clean.py:207-268).
+
+        The reference forms the eigenbasis with a dense eigh and carries the
+        suffix circuit as dense 2^n x 2^n left-hand-side matrices.  For a
+        diagonal (z / zz) observable the eigenbasis is the computational basis
+        sorted by D(x) ascending, so here every shifted circuit runs on the
+        device and is sampled there by the inverse CDF over that order at
+        uniforms drawn on the host in the reference's order (E's shots
+        first when exact_expec_val is False, then per parameter the +pi/2
+        circuit's, then the -pi/2 circuit's).  Observables with x / y terms
+        raise NotImplementedError."""
+        if not self.observable.is_diagonal:
+            raise NotImplementedError('sample_grad_dense: only diagonal (z / zz) observables on the GPU path')
+        D = self._diagonal_values()
+        order = np.argsort(D, kind='stable').astype(np.int32)
+        eig = D[order]
+        terms = self.observable.terms
+        w = np.array([t[3] for t in terms])
+        u_E = None if exact_expec_val else draw_uniforms(len(terms), shot_num)
+        npar = self.lnum * self.qnum
+        u = np.concatenate(draw_uniforms(2 * npar, shot_num)) if npar else np.empty(0)
+        axes, angles = self._params()
+        _keep, ini = self._ini(ini_state, self.qnum)
+        params = np.repeat(np.arange(npar, dtype=np.int32), 2)
+        deltas = np.tile(np.array([np.pi / 2, -np.pi / 2]), npar)
+        idx = np.empty(u.size, dtype=np.int32)
+        _native.call('qg_mcclean_shift_sample', self._get_plan(), _native.iptr(axes), _native.dptr(angles), ini,
+                     2 * npar, _native.iptr(params), _native.dptr(deltas), _native.iptr(order), int(shot_num),
+                     _native.dptr(u), _native.iptr(idx))
+        self._plan_state = True
+        E = self.expec_val() if exact_expec_val else sample_projectors(self._term_probs(terms), w, shot_num,
+                                                                        uniforms=u_E)
+        means = eig[idx].reshape(2 * npar, int(shot_num)).mean(axis=1)
+        grad = ((means[0::2] - means[1::2]) / 2.).reshape(self.lnum, self.qnum)
+        return E, grad
+
+    def sample_grad_dense_with_component_sampling(self, shot_num=1, hide_progbar=True, exact_expec_val=True,
+                                                  ini_state=None):
+        """As sample_grad_dense, measuring one sampled component w_k P_k in
+        its eigenbasis (mc_clean.py:277-350); the component is drawn once
+        per call, before E.  The eigenvalues of w_k P_k are -|w_k| and
+        +|w_k|, each on half of the space, so a shot is -|w_k| when its
+        uniform is at most the mass of the -|w_k| eigenspace (from the device's
+        projector probability of P_k) and +|w_k| otherwise.  Needs
+        ``use_observable_components=True``."""
+        obs = self.observable
+        c = np.random.choice(np.arange(obs.num_components), p=obs.weight_distribution)
+        terms = obs.terms
+        w = np.array([t[3] for t in terms])
+        u_E = None if exact_expec_val else draw_uniforms(len(terms), shot_num)
+        npar = self.lnum * self.qnum
+        u = draw_uniforms(2 * npar, shot_num)
+        params = np.repeat(np.arange(npar, dtype=np.int32), 2)
+        deltas = np.tile(np.array([np.pi / 2, -np.pi / 2]), npar)
+        p_plus = self.shifted_term_probs(params, deltas, [terms[c]], ini_state)[:, 0]
+        E = self.expec_val() if exact_expec_val else sample_projectors(self._term_probs(terms), w, shot_num,
+                                                                        uniforms=u_E)
+        wc = abs(w[c])
+        # mass of the -|w| eigenspace: P_k = -1 for w > 0, +1 for w < 0
+        m_minus = 1.0 - p_plus if w[c] > 0 else p_plus
+        means = np.array([np.where(uk <= m, -wc, wc).mean() for uk, m in zip(u, m_minus)])
+        grad = ((means[0::2] - means[1::2]) / 2.).reshape(self.lnum, self.qnum)
+        return E, grad
+
+    def sample_grad_observable(self, *args):
+        """Renamed in the reference (mc_clean.py:200-205): warns, returns None."""
+        warnings.warn('Method sample_grad_observable is now called sample_grad_dense.', DeprecationWarning,
+                      stacklevel=2)
+
+    def sample_grad_observable_with_component_sampling(self, *args):
+        """Renamed in the reference (mc_clean.py:270-275): warns, returns None."""
+        warnings.warn('Method sample_grad_observable_with_component_sampling is now called '
+                      'sample_grad_dense_with_component_sampling.', DeprecationWarning, stacklevel=2)
 
     def final_state(self):
         """The state after the last forward run (complex128 host copy)."""

@@ -13,7 +13,7 @@ import numpy as np
 
 from .. import _native
 from ..physical_components import Gates
-from .base import ParametrizedCircuit
+from .base import ParametrizedCircuit, sample_projectors
 
 
 class Qaoa(ParametrizedCircuit):
@@ -75,6 +75,98 @@ class Qaoa(ParametrizedCircuit):
         """Not implemented in the reference either (qaoa.py:72-81): warns and
         returns None."""
         warnings.warn('Not implemented yet.')
+
+    def sample_grad_dense(self, betas, gammas, shot_num=1, hide_progbar=True, exact_expec_val=True,
+                          ini_state=None):
+        """(E, grad[p, 2]) from finite-shot measurements of shifted circuits
+        in the computational basis (qaoa.py:83-158).
+
+        For layer i: each classical-Hamiltonian component C_j phase-shifted by
+        exp(-+i pi/4 C_j) after the cost phase gives grad[i, 1] = sum_j
+        (o_plus_j - o_minus_j); each qubit's mixer rotation shifted by +-pi/2
+        gives grad[i, 0] = sum_q (o_plus_q - o_minus_q) / 2.  An estimate o is
+        the mean of C(x) over shot_num basis draws of the shifted circuit's
+        final state.  The reference carries each suffix circuit as a dense
+        left-hand-side matrix; here the shifted states run the suffix on the
+        device from device copies of the forward history, and are sampled
+        there by the inverse CDF at uniforms drawn on the host in the
+        reference's order.  E is <C> (or, with exact_expec_val=False, the
+        shot estimate of the whole observable).  The reference mutates
+        ``ini_state``; it is only read here."""
+        from ..physical_components import State
+        betas, gammas = self.__check_parameters(betas, gammas)
+        n, L = self.qnum, self.lnum
+        cterms = self.observable.classical_terms()
+        arr, k = _native.terms_array(cterms)
+        C = self._classical_values(cterms)
+
+        def new_state():
+            st = State(n, '+', dtype=self.dtype, device=self.device)
+            st.gates = self._gates
+            return st
+
+        st = new_state()
+        if ini_state is not None:
+            st.vec = ini_state
+        hist = []
+        for i in range(L):
+            st.exp_ham_classical(gammas[i])
+            hist.append(new_state().copy_from(st))
+            st.xrot_all(betas[i])
+            hist.append(new_state().copy_from(st))
+        terms = self.observable.terms
+        if exact_expec_val:
+            Eo = ctypes.c_double()
+            _native.call('qg_state_expec_val', st._h, arr, k, ctypes.byref(Eo))
+            E = np.float64(Eo.value)
+        else:
+            E = sample_projectors(st.term_probs(self.observable), [t[3] for t in terms], shot_num)
+        work = new_state()
+        shots = int(shot_num)
+
+        def estimate(first_layer, mixer_first):
+            """Runs the suffix from ``work`` and returns the shot mean of C."""
+            if mixer_first:
+                work.xrot_all(betas[first_layer - 1])
+            for j in range(first_layer, L):
+                work.exp_ham_classical(gammas[j])
+                work.xrot_all(betas[j])
+            u = np.random.uniform(size=shots)
+            return C[work.sample_basis(u)].mean()
+
+        grad = np.empty((L, 2), dtype='double')
+        for i in range(L):
+            dg = 0.
+            for j in range(len(cterms)):
+                out = []
+                for angle in (np.pi / 4, -np.pi / 4):
+                    work.copy_from(hist[2 * i])
+                    work.exp_ham_classical_component(angle, j)
+                    out.append(estimate(i + 1, True))
+                dg += out[0] - out[1]
+            db = 0.
+            for q in range(n):
+                out = []
+                for angle in (np.pi / 2, -np.pi / 2):
+                    work.copy_from(hist[2 * i + 1])
+                    work.xrot(angle, q)
+                    out.append(estimate(i + 1, False))
+                db += out[0] - out[1]
+            grad[i, 0] = .5 * db
+            grad[i, 1] = dg
+        return E, grad
+
+    def _classical_values(self, cterms):
+        """C(x) in natural order, summed in term order (state.py:273-294)."""
+        n = self.qnum
+        x = np.arange(2 ** n, dtype=np.int64)
+        C = np.zeros(2 ** n)
+        for kind, i, j, w in cterms:
+            s = 1.0 - 2.0 * ((x >> (n - 1 - i)) & 1)
+            if kind == _native.TERM_ZZ:
+                s = s * (1.0 - 2.0 * ((x >> (n - 1 - j)) & 1))
+            C = C + w * s
+        return C
 
     def run_expec_val(self, betas, gammas, hide_progbar=True, exact_expec_val=True, shot_num=1, ini_state=None):
         """Forward circuit, then the exact expectation or, with

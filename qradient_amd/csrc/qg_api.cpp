@@ -33,6 +33,8 @@ hipError_t simple_convert_in(int dtype, const void* in_c128, void* out, int n, h
 hipError_t simple_convert_out(int dtype, const void* in, void* out_c128, int n, hipStream_t st);
 hipError_t simple_term_expec(int dtype, const void* v, int n, const PTerm* t, int nt, double* part, double* out,
                              hipStream_t st);
+hipError_t simple_sample_basis(int dtype, const void* v, int n, const int32_t* order, const double* u, int shots,
+                               int32_t* idx, double* mass, hipStream_t st);
 constexpr int kDotBlocks = 512;
 hipError_t launch_fused(const void* descs, int batch, int n, hipStream_t st);
 hipError_t set_fused_attributes();
@@ -1498,25 +1500,21 @@ static void* mcclean_forward_suffix(qg_plan* p, const int32_t* axes, const doubl
   return (void*)cur;
 }
 
-extern "C" int qg_mcclean_shift_term_prob(qg_plan* p, const int32_t* axes, const double* angles,
-                                          const double* ini_state, int nshift, const int32_t* shift_param,
-                                          const double* shift_delta, const qg_term* terms, int nterms,
-                                          double* prob) {
-  if (!p || p->kind != PK_MCCLEAN) return fail(QG_ERR_INVALID, "not a McClean plan");
-  if (nshift < 0 || nterms < 0) return fail(QG_ERR_INVALID, "negative count");
-  if (nshift > 0 && (!shift_param || !shift_delta || (nterms > 0 && !prob)))
-    return fail(QG_ERR_INVALID, "NULL argument");
+// Runs the unshifted circuit through the pass kernels, keeping its layer-end
+// states (history) or checkpoints, then each shifted circuit from the nearest
+// stored state before its shifted layer (the reference restarts from
+// state_history[i], mc_clean.py:140-151), and calls on_shift(s, final state)
+// to enqueue that shift's measurement on the plan's stream.  Afterwards the
+// plan's state is the unshifted circuit's final state.
+template <class OnShift>
+static int mcclean_shift_run(qg_plan* p, const int32_t* axes, const double* angles, const double* ini_state,
+                             int nshift, const int32_t* shift_param, const double* shift_delta, OnShift on_shift) {
   const int n = p->n, L = p->L;
   for (int s = 0; s < nshift; ++s) {
     if (shift_param[s] < 0 || shift_param[s] >= L * n)
       return fail(QG_ERR_INVALID, "shift parameter index out of range");
     if (!std::isfinite(shift_delta[s])) return fail(QG_ERR_INVALID, "non-finite shift");
   }
-  std::vector<PTerm> h;
-  CHK(pterm_table(n, terms, nterms, h));
-  // the unshifted circuit through the pass kernels, keeping its layer-end
-  // states (history) or checkpoints: every shifted circuit starts from the
-  // nearest one before its shifted layer
   const bool fused = p->fused;
   p->fused = false;
   double E = 0.0;
@@ -1524,7 +1522,6 @@ extern "C" int qg_mcclean_shift_term_prob(qg_plan* p, const int32_t* axes, const
   const int r = mcclean_run(p, axes, angles, &none, 1, ini_state, &E, nullptr, false);
   p->fused = fused;
   CHK(r);
-  if (nshift == 0 || nterms == 0) return QG_OK;
   qg_ctx* c = p->ctx;
   const int P = (int)p->geo.size();
   std::vector<const void*> layer_end(L, nullptr);
@@ -1538,19 +1535,8 @@ extern "C" int qg_mcclean_shift_term_prob(qg_plan* p, const int32_t* axes, const
   const void* start0 = ini_state ? wsp(p, p->off_prep) : nullptr;
   // the adjoint's two lambda buffers are free in a forward-only run
   void* buf[2] = {wsp(p, p->off_lam), wsp(p, p->off_lam + p->S)};
-  const int nt = nterms + 1;
-  const size_t tb = ((sizeof(PTerm) * nt + 255) / 256) * 256;
-  const size_t pb = sizeof(double) * kTermBlocks * kTermsPerLaunch;
-  char* dbuf = nullptr;
-  HIPCHK(hipMalloc(&dbuf, tb + pb + sizeof(double) * (size_t)nt * nshift));
-  PTerm* d_t = (PTerm*)dbuf;
-  double* d_part = (double*)(dbuf + tb);
-  double* d_res = (double*)(dbuf + tb + pb);
-  std::vector<double> res((size_t)nt * nshift);
   std::vector<double> ang(angles, angles + (size_t)L * n);
-  int rc = QG_OK;
-  hipError_t e = hipMemcpyAsync(d_t, h.data(), sizeof(PTerm) * nt, hipMemcpyHostToDevice, c->stream);
-  for (int s = 0; s < nshift && e == hipSuccess && rc == QG_OK; ++s) {
+  for (int s = 0; s < nshift; ++s) {
     const int k = shift_param[s], i = k / n;
     int j = i - 1;
     while (j >= 0 && !layer_end[j]) --j;
@@ -1559,23 +1545,104 @@ extern "C" int qg_mcclean_shift_term_prob(qg_plan* p, const int32_t* axes, const
     int nd = 0;
     void* fin = mcclean_forward_suffix(p, axes, ang.data(), j + 1, j >= 0 ? layer_end[j] : start0, buf, nd);
     ang[k] = angles[k];
-    rc = run_launches(p, nd);
-    if (rc != QG_OK) break;
-    e = simple_term_expec(p->dtype, fin, n, d_t, nt, d_part, d_res + (size_t)s * nt, c->stream);
+    CHK(run_launches(p, nd));
+    HIPCHK(on_shift(s, (const void*)fin));
     // the next shift rewrites the pinned descriptor table this one uploads from
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    HIPCHK(hipStreamSynchronize(c->stream));
   }
-  if (e == hipSuccess && rc == QG_OK)
-    e = hipMemcpyAsync(res.data(), d_res, sizeof(double) * res.size(), hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess && rc == QG_OK) e = hipStreamSynchronize(c->stream);
-  (void)hipStreamSynchronize(c->stream);
-  (void)hipFree(dbuf);
-  // the plan's state stays the unshifted circuit's final state
-  if (rc != QG_OK) return rc;
-  if (e != hipSuccess) return fail(QG_ERR_HIP, std::string("shifted circuits: ") + hipGetErrorString(e));
+  return QG_OK;
+}
+
+// Device scratch freed on every return path.
+struct DevBuf {
+  char* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+extern "C" int qg_mcclean_shift_term_prob(qg_plan* p, const int32_t* axes, const double* angles,
+                                          const double* ini_state, int nshift, const int32_t* shift_param,
+                                          const double* shift_delta, const qg_term* terms, int nterms,
+                                          double* prob) {
+  if (!p || p->kind != PK_MCCLEAN) return fail(QG_ERR_INVALID, "not a McClean plan");
+  if (nshift < 0 || nterms < 0) return fail(QG_ERR_INVALID, "negative count");
+  if (nshift > 0 && (!shift_param || !shift_delta || (nterms > 0 && !prob)))
+    return fail(QG_ERR_INVALID, "NULL argument");
+  std::vector<PTerm> h;
+  CHK(pterm_table(p->n, terms, nterms, h));
+  const int nt = nterms + 1;
+  const size_t tb = ((sizeof(PTerm) * nt + 255) / 256) * 256;
+  const size_t pb = sizeof(double) * kTermBlocks * kTermsPerLaunch;
+  HIPCHK(hipSetDevice(p->ctx->device));
+  DevBuf db;
+  HIPCHK(hipMalloc(&db.p, tb + pb + sizeof(double) * (size_t)nt * std::max(nshift, 1)));
+  PTerm* d_t = (PTerm*)db.p;
+  double* d_part = (double*)(db.p + tb);
+  double* d_res = (double*)(db.p + tb + pb);
+  HIPCHK(hipMemcpy(d_t, h.data(), sizeof(PTerm) * nt, hipMemcpyHostToDevice));
+  CHK(mcclean_shift_run(p, axes, angles, ini_state, nterms ? nshift : 0, shift_param, shift_delta,
+                        [&](int s, const void* fin) {
+                          return simple_term_expec(p->dtype, fin, p->n, d_t, nt, d_part, d_res + (size_t)s * nt,
+                                                   p->ctx->stream);
+                        }));
+  if (nshift == 0 || nterms == 0) return QG_OK;
+  std::vector<double> res((size_t)nt * nshift);
+  HIPCHK(hipMemcpy(res.data(), d_res, sizeof(double) * res.size(), hipMemcpyDeviceToHost));
   for (int s = 0; s < nshift; ++s)
     for (int t = 0; t < nterms; ++t)
       prob[(size_t)s * nterms + t] = 0.5 * (res[(size_t)s * nt + nterms] + res[(size_t)s * nt + t]);
+  return QG_OK;
+}
+
+// Device copies of a sampling order (validated: every entry in [0, 2^n)) and
+// of the uniforms, and room for the drawn positions and the block masses.
+struct SampleBufs {
+  DevBuf db;
+  int32_t* order = nullptr;
+  double* u = nullptr;
+  int32_t* idx = nullptr;
+  double* mass = nullptr;
+};
+
+static int sample_setup(int n, const int32_t* order, int64_t nu, const double* u, SampleBufs& b) {
+  if (n > 31) return fail(QG_ERR_UNSUPPORTED, "basis sampling needs n <= 31");
+  const uint64_t N = 1ull << n;
+  if (order)
+    for (uint64_t k = 0; k < N; ++k)
+      if (order[k] < 0 || (uint64_t)order[k] >= N) return fail(QG_ERR_INVALID, "sampling order entry out of range");
+  for (int64_t k = 0; k < nu; ++k)
+    if (!(u[k] >= 0.0 && u[k] <= 1.0)) return fail(QG_ERR_INVALID, "uniform draws must lie in [0, 1]");
+  const size_t nb = (N + kSampleBlock - 1) / kSampleBlock;
+  const size_t ob = order ? ((N * 4 + 255) / 256) * 256 : 0;
+  const size_t ub = (((size_t)nu * 8 + 255) / 256) * 256;
+  const size_t ib = (((size_t)nu * 4 + 255) / 256) * 256;
+  HIPCHK(hipMalloc(&b.db.p, ob + ub + ib + nb * 8));
+  b.order = order ? (int32_t*)b.db.p : nullptr;
+  b.u = (double*)(b.db.p + ob);
+  b.idx = (int32_t*)(b.db.p + ob + ub);
+  b.mass = (double*)(b.db.p + ob + ub + ib);
+  if (order) HIPCHK(hipMemcpy(b.order, order, N * 4, hipMemcpyHostToDevice));
+  if (nu) HIPCHK(hipMemcpy(b.u, u, (size_t)nu * 8, hipMemcpyHostToDevice));
+  return QG_OK;
+}
+
+extern "C" int qg_mcclean_shift_sample(qg_plan* p, const int32_t* axes, const double* angles, const double* ini_state,
+                                       int nshift, const int32_t* shift_param, const double* shift_delta,
+                                       const int32_t* order, int shots, const double* u, int32_t* idx) {
+  if (!p || p->kind != PK_MCCLEAN) return fail(QG_ERR_INVALID, "not a McClean plan");
+  if (nshift < 0 || shots < 1) return fail(QG_ERR_INVALID, "need nshift >= 0 and shots >= 1");
+  if (nshift > 0 && (!shift_param || !shift_delta || !u || !idx)) return fail(QG_ERR_INVALID, "NULL argument");
+  HIPCHK(hipSetDevice(p->ctx->device));
+  SampleBufs b;
+  const int64_t nu = (int64_t)nshift * shots;
+  CHK(sample_setup(p->n, order, nu, u, b));
+  CHK(mcclean_shift_run(p, axes, angles, ini_state, nshift, shift_param, shift_delta,
+                        [&](int s, const void* fin) {
+                          return simple_sample_basis(p->dtype, fin, p->n, b.order, b.u + (size_t)s * shots, shots,
+                                                     b.idx + (size_t)s * shots, b.mass, p->ctx->stream);
+                        }));
+  if (nu) HIPCHK(hipMemcpy(idx, b.idx, (size_t)nu * 4, hipMemcpyDeviceToHost));
   return QG_OK;
 }
 
@@ -2171,6 +2238,32 @@ extern "C" int qg_state_expec_val(qg_state* s, const qg_term* terms, int nterms,
 extern "C" int qg_state_term_prob(qg_state* s, const qg_term* terms, int nterms, double* prob) {
   if (!s) return fail(QG_ERR_INVALID, "NULL state");
   return term_prob(s->ctx, s->dtype, s->v, s->n, terms, nterms, prob);
+}
+
+extern "C" int qg_state_copy(qg_state* dst, const qg_state* src) {
+  if (!dst || !src) return fail(QG_ERR_INVALID, "NULL state");
+  if (dst->n != src->n || dst->dtype != src->dtype)
+    return fail(QG_ERR_INVALID, "qg_state_copy: states differ in qubit number or dtype");
+  if (dst == src) return QG_OK;
+  HIPCHK(hipSetDevice(dst->ctx->device));
+  const size_t bytes = ((size_t)1 << dst->n) * (dst->dtype == QG_C64 ? 8 : 16);
+  // both contexts' earlier work on the two vectors is finished first
+  HIPCHK(hipStreamSynchronize(src->ctx->stream));
+  HIPCHK(hipMemcpyAsync(dst->v, src->v, bytes, hipMemcpyDeviceToDevice, dst->ctx->stream));
+  HIPCHK(hipStreamSynchronize(dst->ctx->stream));
+  return QG_OK;
+}
+
+extern "C" int qg_state_sample_basis(qg_state* st, const int32_t* order, int shots, const double* u, int32_t* idx) {
+  if (!st || !u || !idx) return fail(QG_ERR_INVALID, "NULL argument");
+  if (shots < 1) return fail(QG_ERR_INVALID, "shots must be >= 1");
+  HIPCHK(hipSetDevice(st->ctx->device));
+  SampleBufs b;
+  CHK(sample_setup(st->n, order, shots, u, b));
+  HIPCHK(simple_sample_basis(st->dtype, st->v, st->n, b.order, b.u, shots, b.idx, b.mass, st->ctx->stream));
+  HIPCHK(hipStreamSynchronize(st->ctx->stream));
+  HIPCHK(hipMemcpy(idx, b.idx, (size_t)shots * 4, hipMemcpyDeviceToHost));
+  return QG_OK;
 }
 
 extern "C" int qg_state_norm_error(qg_state* s, double* out) {

@@ -59,6 +59,7 @@ struct PTerm {
 };
 constexpr int kTermsPerLaunch = 16;  // terms measured per pass over the state
 constexpr int kTermBlocks = 2048;    // workgroups of a term pass (partials per term)
+constexpr int kSampleBlock = 1024;   // positions per block of the basis sampler
 
 struct PassDesc {
   // geometry: local bits = [0, c) U [p, p + K - c); tile bits = the rest, in order

@@ -238,6 +238,93 @@ __global__ void k_scale(typename V2<S>::T* v, uint32_t N, double re, double im) 
 // The elementwise kernels count amplitudes in 32 bits: states of up to 2^31.
 constexpr int kSimpleMaxN = 31;
 
+// ---------------------------------------------------------------------------
+// Basis sampling: the inverse CDF of the distribution |v_x|^2 over positions
+// k = 0 .. N-1, position k holding amplitude order[k] (or k without an
+// order).  This is scipy's rv_discrete(values=(arange(N), dist)).rvs as the
+// reference's dense sampled gradients call it (mc_clean.py:246-254,
+// qaoa.py:196-198): a draw u maps to the first position whose cumulative
+// probability reaches u, and to position 0 when none does
+// (argmax(qvals >= u)).  Two levels: masses of 1024-position blocks and
+// their prefix, then a binary search over blocks and a walk inside one.
+// ---------------------------------------------------------------------------
+template <class S>
+__device__ __forceinline__ double prob_at(const typename V2<S>::T* __restrict__ v, const int32_t* __restrict__ order,
+                                          uint32_t k) {
+  const uint32_t x = order ? (uint32_t)order[k] : k;
+  const double re = v[x].x, im = v[x].y;
+  return re * re + im * im;
+}
+
+template <class S>
+__global__ void __launch_bounds__(256) k_block_mass(const typename V2<S>::T* __restrict__ v, uint32_t N,
+                                                    const int32_t* __restrict__ order, double* __restrict__ mass) {
+  __shared__ double red[256];
+  const uint32_t base = blockIdx.x * (uint32_t)kSampleBlock + threadIdx.x * 4u;
+  double s = 0.0;
+  for (uint32_t j = 0; j < 4; ++j)
+    if (base + j < N) s += prob_at<S>(v, order, base + j);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) mass[blockIdx.x] = red[0];
+}
+
+// Inclusive prefix sums of mass[0, nb) in place, one workgroup of 256 threads.
+__global__ void __launch_bounds__(256) k_mass_prefix(double* __restrict__ mass, int nb) {
+  __shared__ double tot[256];
+  const int chunk = (nb + 255) / 256;
+  const int lo = (int)threadIdx.x * chunk, hi = min(nb, lo + chunk);
+  double s = 0.0;
+  for (int b = lo; b < hi; ++b) s += mass[b];
+  tot[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0;
+    for (int t = 0; t < 256; ++t) {
+      a += tot[t];
+      tot[t] = a;
+    }
+  }
+  __syncthreads();
+  double a = threadIdx.x ? tot[threadIdx.x - 1] : 0.0;
+  for (int b = lo; b < hi; ++b) {
+    a += mass[b];
+    mass[b] = a;
+  }
+}
+
+template <class S>
+__global__ void k_basis_search(const typename V2<S>::T* __restrict__ v, uint32_t N, const int32_t* __restrict__ order,
+                               const double* __restrict__ pref, int nb, const double* __restrict__ u, int shots,
+                               int32_t* __restrict__ idx) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= shots) return;
+  const double us = u[s];
+  if (!(pref[nb - 1] >= us)) {
+    idx[s] = 0;
+    return;
+  }
+  int lo = 0, hi = nb - 1;  // the first block whose prefix reaches us
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (pref[mid] >= us) hi = mid;
+    else lo = mid + 1;
+  }
+  double acc = lo ? pref[lo - 1] : 0.0;
+  const uint32_t k0 = (uint32_t)lo * (uint32_t)kSampleBlock;
+  const uint32_t k1 = min(N, k0 + (uint32_t)kSampleBlock);
+  uint32_t k = k0;
+  for (; k < k1; ++k) {
+    acc += prob_at<S>(v, order, k);
+    if (acc >= us) break;
+  }
+  idx[s] = (int32_t)(k < k1 ? k : k1 - 1);  // rounding between the two levels: the block's last position
+}
+
 static inline dim3 grid_for(uint64_t n, int threads) { return dim3((unsigned)((n + threads - 1) / threads)); }
 
 #define QG_DT_SWITCH(dtype, CALL) \
@@ -341,6 +428,21 @@ hipError_t simple_convert_out(int dtype, const void* in, void* out_c128, int n, 
   const uint32_t N = (uint32_t)(1ull << n);
   QG_DT_SWITCH(dtype, hipLaunchKernelGGL(k_convert_out<S>, grid_for(N, 256), dim3(256), 0, st,
                                          (const typename V2<S>::T*)in, (double2*)out_c128, N));
+  return hipGetLastError();
+}
+
+// mass: (2^n + kSampleBlock - 1) / kSampleBlock doubles of scratch.
+hipError_t simple_sample_basis(int dtype, const void* v, int n, const int32_t* order, const double* u, int shots,
+                               int32_t* idx, double* mass, hipStream_t st) {
+  if (n < 1 || n > kSimpleMaxN || shots < 1) return hipErrorInvalidValue;
+  const uint32_t N = (uint32_t)(1ull << n);
+  const int nb = (int)((N + kSampleBlock - 1) / kSampleBlock);
+  QG_DT_SWITCH(dtype, hipLaunchKernelGGL(k_block_mass<S>, dim3(nb), dim3(256), 0, st, (const typename V2<S>::T*)v, N,
+                                         order, mass));
+  hipLaunchKernelGGL(k_mass_prefix, dim3(1), dim3(256), 0, st, mass, nb);
+  QG_DT_SWITCH(dtype, hipLaunchKernelGGL(k_basis_search<S>, grid_for((uint64_t)shots, 256), dim3(256), 0, st,
+                                         (const typename V2<S>::T*)v, N, order, (const double*)mass, nb, u, shots,
+                                         idx));
   return hipGetLastError();
 }
 

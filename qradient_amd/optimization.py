@@ -13,13 +13,11 @@ host.
 (``QaoaBatch``) per step and the same update rule applied to the [B, p, 2]
 parameter block, each circuit with its own state of the rule.
 
-``McCleanOpt.step`` dispatches like the reference (optimization.py:69-91):
-``shot_num == 0`` takes grad_run or, with ``component_sampling``,
-grad_run_with_component_sampling; ``shot_num > 0`` with ``dense_mode=False``
-takes the shot-based parameter-shift gradients sample_grad /
-sample_grad_with_component_sampling.  The dense-eigenbasis variants
-(``dense_mode=True`` with shots: sample_grad_dense*, O(4^n) in the
-reference) raise NotImplementedError, as does QaoaOpt with shots.
+``McCleanOpt.step`` and ``QaoaOpt.step`` dispatch like the reference
+(optimization.py:69-91, 113-129): ``shot_num == 0`` takes grad_run (McClean:
+grad_run_with_component_sampling with ``component_sampling``); ``shot_num >
+0`` takes the shot-based gradients, sample_grad_dense* with ``dense_mode``
+(the default) and, for McClean, sample_grad* without.
 """
 import numpy as np
 
@@ -55,9 +53,6 @@ class ParametrizedCircuitOptimizer:
             print('Maximum amount of iterations reached: {}.'.format(self.max_iter))
 
 
-def _no_dense(what):
-    raise NotImplementedError('{}: the dense-eigenbasis sampled gradients are not built; use '
-                              'dense_mode=False (McClean) or shot_num=0'.format(what))
 
 
 class McCleanOpt(ParametrizedCircuitOptimizer):
@@ -85,7 +80,10 @@ class McCleanOpt(ParametrizedCircuitOptimizer):
             else:
                 e, g = self.circuit.grad_run()
         elif dense_mode:
-            _no_dense('sample_grad_dense')
+            if component_sampling:
+                e, g = self.circuit.sample_grad_dense_with_component_sampling(shot_num=shot_num)
+            else:
+                e, g = self.circuit.sample_grad_dense(shot_num=shot_num)
         elif component_sampling:
             e, g = self.circuit.sample_grad_with_component_sampling(shot_num=shot_num)
         else:
@@ -119,11 +117,13 @@ class QaoaOpt(ParametrizedCircuitOptimizer):
 
     def step(self, shot_num=0, dense_mode=True):
         self._check_iter()
-        if shot_num != 0:
-            if not dense_mode:
-                raise ValueError('dense_mode must be True, sparse sampling method is not implmented yet.')
-            _no_dense('Qaoa.sample_grad_dense')
-        e, g = self.circuit.grad_run(self.param_history[self.iter, :, 0], self.param_history[self.iter, :, 1])
+        betas, gammas = self.param_history[self.iter, :, 0], self.param_history[self.iter, :, 1]
+        if shot_num == 0:
+            e, g = self.circuit.grad_run(betas, gammas)
+        elif dense_mode:
+            e, g = self.circuit.sample_grad_dense(betas, gammas, shot_num=shot_num)
+        else:
+            raise ValueError('dense_mode must be True, sparse sampling method is not implmented yet.')
         self.cost_history[self.iter] = e
         self.optimizer.step(g, e)
         self.iter += 1

@@ -53,6 +53,26 @@ class State:
             raise ValueError('state vector must have shape ({},)'.format(2 ** self.qnum))
         _native.call('qg_state_set', self._h, v.view(np.float64).ctypes.data_as(_native._dp))
 
+    def copy_from(self, other):
+        """vec <- other.vec on the device (same qubit number and dtype)."""
+        _native.call('qg_state_copy', self._h, other._h)
+        return self
+
+    def sample_basis(self, uniforms, order=None):
+        """Basis positions drawn at the given uniforms: the inverse CDF of
+        |vec|^2 over the positions of ``order`` (natural order if None), as
+        scipy's rv_discrete(values=(arange(2^n), dist)).rvs maps its uniforms
+        (qaoa.py:196-198).  Returns int32 positions."""
+        u = np.ascontiguousarray(uniforms, dtype=np.float64).ravel()
+        idx = np.empty(u.size, dtype=np.int32)
+        o = None if order is None else np.ascontiguousarray(order, dtype=np.int32)
+        if o is not None and o.shape != (2 ** self.qnum,):
+            raise ValueError('order must have shape ({},)'.format(2 ** self.qnum))
+        if u.size:
+            _native.call('qg_state_sample_basis', self._h, None if o is None else _native.iptr(o), int(u.size),
+                         _native.dptr(u), _native.iptr(idx))
+        return idx
+
     def reset(self, ini=None):
         """state.py:61-71; ``reset('+')`` also changes the stored initial state (qaoa.py:17)."""
         if ini is not None:

@@ -140,6 +140,8 @@ class TracingMcClean(QuadMcClean):
     grad_run_with_component_sampling = _traced('grad_run_with_component_sampling')
     sample_grad = _traced('sample_grad')
     sample_grad_with_component_sampling = _traced('sample_grad_with_component_sampling')
+    sample_grad_dense = _traced('sample_grad_dense')
+    sample_grad_dense_with_component_sampling = _traced('sample_grad_dense_with_component_sampling')
 
 
 def test_step_dispatches_like_the_reference():
@@ -148,18 +150,30 @@ def test_step_dispatches_like_the_reference():
     opt.step(component_sampling=True)
     opt.step(shot_num=10, dense_mode=False)
     opt.step(shot_num=5, dense_mode=False, component_sampling=True)
+    opt.step(shot_num=3)
+    opt.step(shot_num=4, component_sampling=True)
     assert circ.calls == [('grad_run_with_component_sampling', {}), ('sample_grad', {'shot_num': 10}),
-                          ('sample_grad_with_component_sampling', {'shot_num': 5})]
-    assert opt.iter == 3
+                          ('sample_grad_with_component_sampling', {'shot_num': 5}),
+                          ('sample_grad_dense', {'shot_num': 3}),
+                          ('sample_grad_dense_with_component_sampling', {'shot_num': 4})]
+    assert opt.iter == 5
 
 
-def test_dense_shot_gradients_are_not_built():
-    opt = McCleanOpt(QuadMcClean(1, 2), {'name': 'Adam'}, max_iter=3)
-    with pytest.raises(NotImplementedError):
-        opt.step(shot_num=10)
-    qopt = QaoaOpt(QuadQaoa(2), {'name': 'Adam'}, np.zeros(2), np.zeros(2), max_iter=3)
-    with pytest.raises(NotImplementedError):
-        qopt.step(shot_num=10)
+class TracingQaoa(QuadQaoa):
+    def __init__(self, *a, **k):
+        super().__init__(*a, **k)
+        self.calls = []
+
+    def sample_grad_dense(self, betas, gammas, shot_num=1):
+        self.calls.append(('sample_grad_dense', shot_num))
+        return self.grad_run(betas, gammas)
+
+
+def test_qaoa_step_dispatches_like_the_reference():
+    circ = TracingQaoa(2)
+    qopt = QaoaOpt(circ, {'name': 'Adam'}, np.zeros(2), np.zeros(2), max_iter=3)
+    qopt.step(shot_num=10)
+    assert circ.calls == [('sample_grad_dense', 10)]
     with pytest.raises(ValueError):
         qopt.step(shot_num=10, dense_mode=False)
 

@@ -373,6 +373,11 @@ struct qg_plan {
   bool use_graph = false;
   hipGraphExec_t gexec = nullptr;
   std::vector<int64_t> gsig;
+  // McClean with x / y observable terms on the pass kernels: lambda = O psi
+  // and E = Re<psi|lambda> by the elementwise kernels (a kind-1 launch)
+  const void* gen_in = nullptr;
+  void* gen_out = nullptr;
+  int gen_ndiag = 0, gen_nxy = 0;
   int obs_desc = -1;     // McClean: descriptor whose row K holds E
   int adj_desc = -1;     // McClean: first adjoint descriptor
   bool state_sign = false;  // the pass kernels' final state is -1 times the circuit's
@@ -764,9 +769,24 @@ static void drop_graph(qg_plan* p) {
   p->gsig.clear();
 }
 
+// lambda = O psi with x / y terms (k_obs_apply) and E = Re<psi|lambda> into
+// d_dot[kDotBlocks]: the observable step of a McClean run whose observable
+// the pass kernels' diagonal form cannot carry.
+static hipError_t launch_generic_obs(qg_plan* p) {
+  hipError_t e = simple_obs_apply(p->dtype, p->gen_in, p->gen_out, p->n, p->d_terms, p->gen_ndiag, p->d_xy,
+                                  p->gen_nxy, p->ctx->stream);
+  if (e == hipSuccess)
+    e = simple_dot(p->dtype, p->gen_in, p->gen_out, p->n, 0, p->d_dot, p->d_dot + kDotBlocks, p->ctx->stream);
+  return e;
+}
+
 static hipError_t launch_all(qg_plan* p) {
   qg_ctx* c = p->ctx;
   for (const Launch& l : p->launches) {
+    if (l.kind == 1) {
+      const hipError_t e = launch_generic_obs(p);
+      if (e != hipSuccess) return e;
+    }
     if (l.kind != 0) continue;
     const hipError_t e = launch_pass(kdtype(p), l.R, l.two, l.phase, l.post, p->d_desc + l.desc, l.K, l.grid,
                                      l.batch, l.threads, p->n, c->stream, nullptr, l.tab);
@@ -820,6 +840,11 @@ static int run_launches(qg_plan* p, int ndesc) {
     sig.push_back(kdtype(p));
     sig.push_back(p->n);
     sig.push_back((int64_t)(intptr_t)p->d_desc);
+    // the elementwise observable launch bakes its buffers and term counts in
+    sig.push_back((int64_t)(intptr_t)p->gen_in);
+    sig.push_back((int64_t)(intptr_t)p->gen_out);
+    sig.push_back(p->gen_ndiag);
+    sig.push_back(p->gen_nxy);
     for (const Launch& l : p->launches) {
       const int64_t v[11] = {l.kind, l.R, l.two, l.phase, l.post, l.desc, l.K, l.grid, l.batch, l.threads, l.tab};
       sig.insert(sig.end(), v, v + 11);
@@ -863,6 +888,8 @@ static int run_launches(qg_plan* p, int ndesc) {
     if (l.kind == 0) {
       HIPCHK(launch_pass(kdtype(p), l.R, l.two, l.phase, l.post, p->d_desc + l.desc, l.K, l.grid, l.batch,
                          l.threads, p->n, c->stream, nullptr, l.tab));
+    } else if (l.kind == 1) {
+      HIPCHK(launch_generic_obs(p));
     }
   }
   if (p->profile) HIPCHK(hipEventRecord(p->pev[nl], c->stream));
@@ -877,6 +904,10 @@ static int collect_profile(qg_plan* p) {
   HIPCHK(hipEventSynchronize(p->pev[nl]));
   for (size_t k = 0; k < nl; ++k) {
     HIPCHK(hipEventElapsedTime(&p->pass_ms[k], p->pev[k], p->pev[k + 1]));
+    if (p->launches[k].kind != 0) {
+      p->pass_kind[k] = 3;  // not a pass kernel
+      continue;
+    }
     const PassDesc& d = p->h_desc[p->launches[k].desc];
     p->pass_kind[k] = (d.flags & F_REDUCE) ? 1 : (d.flags & F_OBS_DIAG) ? 2 : 0;
   }
@@ -1158,6 +1189,20 @@ static int mcclean_schedule(qg_plan* p, const int32_t* axes, const double* angle
       cur = out;
     }
   }
+  if (!diag_obs) {
+    // x / y terms: lambda = O psi and E by the elementwise kernels
+    Launch l;
+    l.desc = -1;
+    l.R = l.K = l.threads = l.grid = l.phase = l.tab = 0;
+    l.post = l.two = false;
+    l.batch = 1;
+    l.kind = 1;
+    p->launches.push_back(l);
+    p->gen_in = cur;
+    p->gen_out = lam[0];
+    p->gen_ndiag = (int)obs.diag.size();
+    p->gen_nxy = (int)obs.xy.size();
+  }
   if (p->obs_desc < 0 && diag_obs) {
     // lambda = D psi and E in a gate-free pass of its own
     const Geo og = obs_geo(n);
@@ -1372,11 +1417,13 @@ static int mcclean_run(qg_plan* p, const int32_t* axes, const double* angles, co
     p->state_sign = false;
     return QG_OK;
   }
-  if (!obs.xy.empty())
-    return fail(QG_ERR_UNSUPPORTED, "observables with x/y terms need n <= 13 (fused path) in this build");
+  if (!obs.xy.empty() && n > 31) return fail(QG_ERR_UNSUPPORTED, "observables with x/y terms need n <= 31");
   CHK(ensure_workspace(p));
   if (!obs.diag.empty())
     HIPCHK(hipMemcpyAsync(p->d_terms, obs.diag.data(), sizeof(DTerm) * obs.diag.size(), hipMemcpyHostToDevice,
+                          c->stream));
+  if (!obs.xy.empty())
+    HIPCHK(hipMemcpyAsync(p->d_xy, obs.xy.data(), sizeof(DTerm) * obs.xy.size(), hipMemcpyHostToDevice,
                           c->stream));
   p->h_terms = obs.diag;
   if (ini_state) {
@@ -1398,14 +1445,17 @@ static int mcclean_run(qg_plan* p, const int32_t* axes, const double* angles, co
   std::vector<std::vector<int>> out_rows(grad ? 1 + L * n : 1);
   std::vector<int> cnt(p->nrows, 0);
   for (const Launch& l : p->launches) {
+    if (l.kind != 0) continue;  // no descriptor
     const PassDesc& d = p->h_desc[l.desc];
     if (d.row_base < 0) continue;
     for (int k = 0; k <= kMaxK; ++k) cnt[d.row_base + k] = l.grid;
   }
   {
-    // E: row K of the observable pass
-    const PassDesc& d = p->h_desc[p->obs_desc];
-    out_rows[0].push_back(d.row_base + d.K);
+    // E: row K of the observable pass (none with x / y terms: from d_dot)
+    if (p->obs_desc >= 0) {
+      const PassDesc& d = p->h_desc[p->obs_desc];
+      out_rows[0].push_back(d.row_base + d.K);
+    }
     if (grad) {
       int idx = p->adj_desc;
       for (int i = L - 1; i >= 0; --i) {
@@ -1431,6 +1481,7 @@ static int mcclean_run(qg_plan* p, const int32_t* axes, const double* angles, co
   HIPCHK(hipMemcpyAsync(p->h_out, p->d_out, sizeof(double) * nout, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   *E_out = p->h_out[0];
+  if (p->obs_desc < 0) HIPCHK(hipMemcpy(E_out, p->d_dot + kDotBlocks, sizeof(double), hipMemcpyDeviceToHost));
   if (grad) std::memcpy(grad_out, p->h_out + 1, sizeof(double) * L * n);
   return QG_OK;
 }

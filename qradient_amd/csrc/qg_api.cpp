@@ -320,6 +320,8 @@ struct qg_plan {
   bool lds64 = false;
   int chunk = 1;  // circuits resident at once (QAOA batches)
   bool fused = true;       // n <= 13: whole circuit per workgroup (qg_fused.hip)
+  char* h_stage = nullptr;  // pinned staging of QAOA term and phase tables
+  size_t stage_cap = 0;
   char* h_arena = nullptr;  // per-call tables, pinned host
   char* d_arena = nullptr;  // and their device copy
   size_t arena_cap = 0;
@@ -656,6 +658,28 @@ static bool set_gate(PassDesc& d, int lb, int axis, double angle, double sign) {
   return flipped;
 }
 
+// The (ga, gb) pair set_gate stores for a gate of this angle and sign.
+struct GateAB {
+  double a, b;
+};
+static GateAB gate_ab(double angle, double sign) {
+  const double phi = std::remainder(sign * 0.5 * angle, M_PI);
+  const double c = std::cos(phi), s = std::sin(phi);
+  return {s / (1.0 + c), s};
+}
+
+// Pinned staging for per-call tables, grown as needed; the previous call's
+// copies out of it have completed (every run ends with a stream sync).
+static int ensure_stage(qg_plan* p, size_t bytes) {
+  if (bytes <= p->stage_cap) return QG_OK;
+  if (p->h_stage) (void)hipHostFree(p->h_stage);
+  p->h_stage = nullptr;
+  p->stage_cap = 0;
+  HIPCHK(hipHostMalloc(&p->h_stage, bytes, hipHostMallocDefault));
+  p->stage_cap = bytes;
+  return QG_OK;
+}
+
 // Fills the per-stage-slot gate tables of a descriptor from its per-bit
 // arrays, for a launch holding 2^R amplitudes per thread (the kernel's stage
 // layout: stage 0 holds local bits [t, t + R), stage s >= 1 holds
@@ -955,8 +979,11 @@ static void free_small(qg_plan* p) {
   if (p->h_desc) (void)hipHostFree(p->h_desc);
   if (p->h_out) (void)hipHostFree(p->h_out);
   if (p->h_arena) (void)hipHostFree(p->h_arena);
+  if (p->h_stage) (void)hipHostFree(p->h_stage);
   (void)hipFree(p->d_arena);
   (void)hipFree(p->d_fstate);
+  p->h_stage = nullptr;
+  p->stage_cap = 0;
   p->h_arena = nullptr;
   p->d_arena = nullptr;
   p->d_fstate = nullptr;
@@ -1837,16 +1864,21 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
   float total_ms = 0.f;
   for (int c0 = 0; c0 < B; c0 += chunk) {
     const int nb = std::min(chunk, B - c0);
-    // terms and phase tables of this chunk
-    std::vector<DTerm> tbuf((size_t)p->max_terms * nb);
+    // terms and phase tables of this chunk, staged in pinned memory and
+    // copied on the stream ahead of the passes
+    const size_t term_bytes = sizeof(DTerm) * (size_t)p->max_terms * nb;
+    const size_t lut_doubles = phase == 1 ? (size_t)nb * 2 * L * lut_len * 2 : 0;
+    CHK(ensure_stage(p, term_bytes + sizeof(double) * lut_doubles));
+    DTerm* tbuf = (DTerm*)p->h_stage;
     std::vector<int> toff(nb), tcnt(nb);
+    p->h_terms.assign((size_t)p->max_terms * nb, DTerm{});
     for (int j = 0; j < nb; ++j) {
       toff[j] = j * p->max_terms;
       tcnt[j] = (int)obs[c0 + j].diag.size();
-      std::copy(obs[c0 + j].diag.begin(), obs[c0 + j].diag.end(), tbuf.begin() + toff[j]);
+      std::copy(obs[c0 + j].diag.begin(), obs[c0 + j].diag.end(), p->h_terms.begin() + toff[j]);
     }
-    HIPCHK(hipMemcpy(p->d_terms, tbuf.data(), sizeof(DTerm) * tbuf.size(), hipMemcpyHostToDevice));
-    p->h_terms = tbuf;
+    std::memcpy(tbuf, p->h_terms.data(), term_bytes);
+    HIPCHK(hipMemcpyAsync(p->d_terms, tbuf, term_bytes, hipMemcpyHostToDevice, c->stream));
     if (ini) {
       // upload (complex128) and convert to the storage type
       const size_t N = 1ull << n;
@@ -1862,21 +1894,30 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
       (void)hipFree(tmp);
       if (e != hipSuccess) return fail(QG_ERR_HIP, std::string("ini_state upload: ") + hipGetErrorString(e));
     }
-    std::vector<double> lut;
     if (phase == 1) {
-      lut.resize((size_t)nb * 2 * L * lut_len * 2);
+      // t[k] = exp(-i g (k - W)) for g = +gamma (dir 0) and -gamma (dir 1).
+      // cos is even and sin odd (exactly, in the math library), so one
+      // sincos per |d| fills four entries
+      double* lut = (double*)(p->h_stage + term_bytes);
       for (int j = 0; j < nb; ++j)
-        for (int i = 0; i < L; ++i)
-          for (int dir = 0; dir < 2; ++dir) {
-            const double g = (dir == 0 ? 1.0 : -1.0) * gammas[(c0 + j) * L + i];
-            double* t = lut.data() + (((size_t)j * L + i) * 2 + dir) * lut_len * 2;
-            for (int k = 0; k < lut_len; ++k) {
-              const double dv = (double)(k - Wi);
-              t[2 * k] = std::cos(g * dv);
-              t[2 * k + 1] = -std::sin(g * dv);
-            }
+        for (int i = 0; i < L; ++i) {
+          const double g = gammas[(c0 + j) * L + i];
+          double* t0 = lut + (((size_t)j * L + i) * 2 + 0) * lut_len * 2;
+          double* t1 = lut + (((size_t)j * L + i) * 2 + 1) * lut_len * 2;
+          for (int m = 0; m <= Wi; ++m) {
+            const double cm = std::cos(g * (double)m), sm = std::sin(g * (double)m);
+            // dir 0 at d = +m and d = -m; dir 1 is its mirror image
+            t0[2 * (Wi + m)] = cm;
+            t0[2 * (Wi + m) + 1] = -sm;
+            t0[2 * (Wi - m)] = cm;
+            t0[2 * (Wi - m) + 1] = sm;
+            t1[2 * (Wi + m)] = cm;
+            t1[2 * (Wi + m) + 1] = sm;
+            t1[2 * (Wi - m)] = cm;
+            t1[2 * (Wi - m) + 1] = -sm;
           }
-      HIPCHK(hipMemcpy(p->d_lut, lut.data(), sizeof(double) * lut.size(), hipMemcpyHostToDevice));
+        }
+      HIPCHK(hipMemcpyAsync(p->d_lut, lut, sizeof(double) * lut_doubles, hipMemcpyHostToDevice, c->stream));
     }
     auto lut_ptr = [&](int j, int i, int dir) -> const double* {
       return phase == 1 ? p->d_lut + (((size_t)j * L + i) * 2 + dir) * lut_len * 2 : nullptr;
@@ -1885,6 +1926,13 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
       return wsp(p, p->off_hist + (((size_t)j * L + i) * P + g) * p->S);
     };
     auto lam = [&](int j) { return wsp(p, p->off_lam + (size_t)j * 2 * p->S); };
+    // every qubit's mixer gate of circuit j, layer i: forward and inverse
+    std::vector<GateAB> gab((size_t)nb * L * 2);
+    for (int j = 0; j < nb; ++j)
+      for (int i = 0; i < L; ++i) {
+        gab[((size_t)j * L + i) * 2] = gate_ab(betas[(c0 + j) * L + i], 1.0);
+        gab[((size_t)j * L + i) * 2 + 1] = gate_ab(betas[(c0 + j) * L + i], -1.0);
+      }
     // descriptors: one batched launch per pass, nb consecutive descriptors
     p->launches.clear();
     int nd = 0, row = 0;
@@ -1955,7 +2003,12 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
           d.a_in = (i == 0 && g == 0) ? (ini ? (const void*)((char*)p->d_ini_states + (size_t)j * p->S) : nullptr)
                                       : (g == 0 ? hist(j, i - 1, P - 1) : hist(j, i, g - 1));
           d.a_out = hist(j, i, g);
-          for (int lb = G.own_lo; lb < G.own_hi; ++lb) set_gate(d, lb, QG_AXIS_X, betas[(c0 + j) * L + i], 1.0);
+          const GateAB ab = gab[((size_t)j * L + i) * 2];
+          for (int lb = G.own_lo; lb < G.own_hi; ++lb) {
+            d.gtype[lb] = G_X;
+            d.ga[lb] = ab.a;
+            d.gb[lb] = ab.b;
+          }
           if (last) {
             f |= F_OBS_DIAG | F_STORE_B;
             d.b_out = lam(j);
@@ -1980,8 +2033,11 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
             d.b_in = lam(j);
             d.b_out = lam(j);
             d.row_base = row;
+            const GateAB ab = gab[((size_t)j * L + i) * 2 + 1];
             for (int lb = G.own_lo; lb < G.own_hi; ++lb) {
-              set_gate(d, lb, QG_AXIS_X, betas[(c0 + j) * L + i], -1.0);
+              d.gtype[lb] = G_X;
+              d.ga[lb] = ab.a;
+              d.gb[lb] = ab.b;
               d.own[lb] = 1;
               out_rows[(size_t)j * per_out + 1 + 2 * i].push_back(row + lb);
             }

@@ -222,16 +222,81 @@ def bench_large(args, sync):
                  c64_vs_c128_grad_rel_err=rel, effective_gbs=b_alg / (c64['ms'] * 1e-3) / 1e9, b_alg_bytes=b_alg)
 
 
+def bench_sample(args, sync):
+    """McClean.sample_grad (shot-based parameter shift): 24 qubits, 8 layers, complex64, 100 shots per term.
+
+    One step is one sample_grad call: the unshifted circuit, then 2 * 8 * 24 = 384 shifted circuits, each
+    restarted from the stored state after the layer before its shifted one and measured for the 23 ZZ terms
+    on the device; the shots are drawn on the host.  Algorithmic bytes per call: 2 S per layer of every
+    circuit run (one read and one write of the state per layer, the ideal single pass) and S per 16 terms
+    measured, with S = 2^24 * 8 bytes."""
+    from qradient_amd.circuit_logic import McClean
+    n, L, shots = 24, 8, 100
+    rs = np.random.RandomState(SEED)
+    axes = (3 * rs.rand(L, n)).astype(int)
+    angles = 2 * np.pi * rs.rand(L, n)
+    zz = np.full((n, n), None)
+    for i, w in enumerate(rs.standard_normal(n - 1)):
+        zz[i, i + 1] = float(w)
+    obs = {'zz': zz}
+    circ = McClean(n, obs, L, axes=axes, angles=angles, dtype='complex64')
+    for _ in range(max(1, args.warmup)):
+        circ.sample_grad(shot_num=shots)
+    steps = max(1, args.steps // 2)
+    sync()
+    t0 = time.perf_counter()
+    dev = 0.0
+    for _ in range(steps):
+        np.random.seed(SEED)
+        E, g = circ.sample_grad(shot_num=shots)
+        dev += circ.last_device_ms()
+    sync()
+    dt = (time.perf_counter() - t0) / steps
+    dev /= steps
+    S = (2 ** n) * 8
+    layers = L + sum(2 * n * (L - i) for i in range(L))
+    launches_per_shift = -(-(n - 1) // 16)
+    b_alg = 2 * layers * S + 2 * n * L * launches_per_shift * S
+    achieved = b_alg / (dev * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        # one layer of one shifted circuit in the numpy oracle, scaled by the layers of the call
+        from oracle import statevector as sv
+        v = sv.mcclean_forward(n, axes[:1], angles[:1])
+        t1 = time.perf_counter()
+        sv.entangle(v, 'cnot')
+        for q in range(n):
+            sv.rot(v, axes[1, q], angles[1, q], q)
+        t_layer = time.perf_counter() - t1
+        cpu = {'value': 1.0 / (t_layer * layers), 'unit': 'sample_grad calls/s', 'cores': 1, 'kind': 'port',
+               'sample': 'one layer of one shifted 24-qubit circuit in the numpy oracle (oracle/statevector.py), '
+                         '%.2f s, scaled x%d to the layers of one call; term measurement and shots excluded'
+                         % (t_layer, layers)}
+    return _line('sample_grad calls/s (McClean 24 qubits, 8 layers, complex64, parameter shift with 100 shots '
+                 'per term)', 1.0 / dt, 'sample_grad calls/s', steps, args.warmup, dt * 1e3,
+                 'c64 (FP64 arithmetic and reductions)',
+                 {'workload': 'mcclean_sample_grad_n24_L8_zzchain_shots100', 'qubits': n, 'layers': L,
+                  'shifted_circuits': 2 * n * L, 'shots': shots, 'seed': SEED},
+                 device_ms_per_step=dev,
+                 roofline={'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                           'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
+                           'kernel': 'whole call (forward passes of every circuit, term measurements)',
+                           'algorithmic_bytes_per_step': b_alg},
+                 cpu_baseline=cpu, E=float(E))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=10)
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--no-cpu-baseline', action='store_true')
-    ap.add_argument('--workload', default='mcclean24', choices=['mcclean24', 'qaoa512', 'serve', 'mcclean30'],
+    ap.add_argument('--workload', default='mcclean24',
+                    choices=['mcclean24', 'qaoa512', 'serve', 'mcclean30', 'sample24'],
                     help='mcclean24 (default, the headline line); qaoa512: 512 batched QAOA MaxCut grad_runs '
                          '(20 qubits, p=8); serve: Poisson request stream through the serving loop; mcclean30: '
-                         'McClean 30 qubits, 100 layers, complex64 checkpoints vs complex128')
+                         'McClean 30 qubits, 100 layers, complex64 checkpoints vs complex128; sample24: '
+                         'shot-based parameter-shift gradient, 24 qubits, 8 layers')
     ap.add_argument('--probe', action='store_true',
                     help='short run for hardware-counter collection: two complex64 grad_runs of the workload '
                          'and one plain 1 GiB device copy (the calibration kernel for FETCH_SIZE / WRITE_SIZE)')
@@ -248,8 +313,8 @@ def main():
         torch.cuda.synchronize()
 
     if args.workload != 'mcclean24':
-        print(json.dumps({'qaoa512': bench_qaoa, 'serve': bench_serve, 'mcclean30': bench_large}[args.workload](
-            args, sync)))
+        print(json.dumps({'qaoa512': bench_qaoa, 'serve': bench_serve, 'mcclean30': bench_large,
+                          'sample24': bench_sample}[args.workload](args, sync)))
         return
     axes, angles, obs = workload()
     if args.probe:

@@ -22,10 +22,29 @@ def progbar_range(hide_progbar):
 
 
 def draw_uniforms(count, shot_num, random_state=None):
-    """``count`` arrays of shot_num uniforms, drawn as ``count`` successive
-    rv_discrete.rvs calls draw them (one ``uniform(size=shot_num)`` each)."""
+    """[count, shot_num] uniforms: the numbers ``count`` successive
+    rv_discrete.rvs calls draw (one ``uniform(size=shot_num)`` each; the
+    legacy RandomState fills one larger request in the same order)."""
     rs = np.random.mtrand._rand if random_state is None else random_state
-    return [rs.uniform(size=int(shot_num)) for _ in range(count)]
+    return rs.uniform(size=(int(count), int(shot_num)))
+
+
+def projector_estimates(probs, weights, uniforms):
+    """sample_projectors for many states at once: probs [M, K], uniforms
+    [M, K, shots] in draw order; returns the M estimates, each summed over
+    its K terms in order (the reference's accumulation, base.py:27-33)."""
+    probs = np.clip(np.asarray(probs, dtype=np.float64), 0.0, 1.0)
+    q1 = probs + (1. - probs)   # np.cumsum([p, 1 - p])[1]
+    u = np.asarray(uniforms)
+    first = u <= probs[..., None]
+    second = u <= q1[..., None]
+    w = np.asarray(weights, dtype=np.float64)[None, :, None]
+    vals = np.where(first | ~second, w, -w)
+    means = vals.mean(axis=-1)
+    total = np.zeros(means.shape[0])
+    for k in range(means.shape[1]):
+        total = total + means[:, k]
+    return total
 
 
 def sample_projectors(probs, weights, shot_num, random_state=None, uniforms=None):

@@ -20,7 +20,7 @@ import numpy as np
 
 from .. import _native
 from ..physical_components import Gates
-from .base import ParametrizedCircuit, draw_uniforms, sample_projectors
+from .base import ParametrizedCircuit, draw_uniforms, projector_estimates, sample_projectors
 
 
 class McClean(ParametrizedCircuit):
@@ -153,6 +153,8 @@ class McClean(ParametrizedCircuit):
         E (exact, or its shot estimate), then the probabilities of every term
         in the +pi/2 and -pi/2 shifted circuit of every parameter, rows in
         the reference's loop order (layer, qubit, then +, -)."""
+        if int(shot_num) < 1:
+            raise ValueError('shot_num must be at least 1')
         npar = self.lnum * self.qnum
         params = np.repeat(np.arange(npar, dtype=np.int32), 2)
         deltas = np.tile(np.array([np.pi / 2, -np.pi / 2]), npar)
@@ -175,11 +177,12 @@ class McClean(ParametrizedCircuit):
         parameter o_plus's, then o_minus's), so a seeded call gives the
         reference's estimate."""
         E, probs, w = self._shift_run(shot_num, exact_expec_val, ini_state)
-        grad = np.empty((self.lnum, self.qnum), dtype='double')
-        for k in range(self.lnum * self.qnum):
-            o_plus = sample_projectors(probs[2 * k], w, shot_num)
-            o_minus = sample_projectors(probs[2 * k + 1], w, shot_num)
-            grad.flat[k] = .5 * (o_plus - o_minus)
+        npar = self.lnum * self.qnum
+        # every shifted circuit's draws in the reference's order: per
+        # parameter o_plus's terms, then o_minus's, shot_num each
+        u = np.random.uniform(size=(2 * npar, len(w), int(shot_num))) if npar and len(w) else None
+        o = projector_estimates(probs, w, u) if u is not None else np.zeros(2 * npar)
+        grad = (.5 * (o[0::2] - o[1::2])).reshape(self.lnum, self.qnum)
         return E, grad
 
     def sample_grad_with_component_sampling(self, hide_progbar=True, shot_num=1, exact_expec_val=True,
@@ -229,6 +232,8 @@ class McClean(ParametrizedCircuit):
         raise NotImplementedError."""
         if not self.observable.is_diagonal:
             raise NotImplementedError('sample_grad_dense: only diagonal (z / zz) observables on the GPU path')
+        if int(shot_num) < 1:
+            raise ValueError('shot_num must be at least 1')
         D = self._diagonal_values()
         order = np.argsort(D, kind='stable').astype(np.int32)
         eig = D[order]
@@ -236,7 +241,7 @@ class McClean(ParametrizedCircuit):
         w = np.array([t[3] for t in terms])
         u_E = None if exact_expec_val else draw_uniforms(len(terms), shot_num)
         npar = self.lnum * self.qnum
-        u = np.concatenate(draw_uniforms(2 * npar, shot_num)) if npar else np.empty(0)
+        u = draw_uniforms(2 * npar, shot_num).ravel()
         axes, angles = self._params()
         _keep, ini = self._ini(ini_state, self.qnum)
         params = np.repeat(np.arange(npar, dtype=np.int32), 2)
@@ -261,6 +266,8 @@ class McClean(ParametrizedCircuit):
         uniform is at most the mass of the -|w_k| eigenspace (from the device's
         projector probability of P_k) and +|w_k| otherwise.  Needs
         ``use_observable_components=True``."""
+        if int(shot_num) < 1:
+            raise ValueError('shot_num must be at least 1')
         obs = self.observable
         c = np.random.choice(np.arange(obs.num_components), p=obs.weight_distribution)
         terms = obs.terms

@@ -32,7 +32,7 @@ hipError_t simple_scale(int dtype, void* v, int n, double re, double im, hipStre
 hipError_t simple_convert_in(int dtype, const void* in_c128, void* out, int n, hipStream_t st);
 hipError_t simple_convert_out(int dtype, const void* in, void* out_c128, int n, hipStream_t st);
 hipError_t simple_term_expec(int dtype, const void* v, int n, const PTerm* t, int nt, double* part, double* out,
-                             hipStream_t st);
+                             hipStream_t st, const PTerm* h_t);
 hipError_t simple_sample_basis(int dtype, const void* v, int n, const int32_t* order, const double* u, int shots,
                                int32_t* idx, double* mass, hipStream_t st);
 constexpr int kDotBlocks = 512;
@@ -562,7 +562,7 @@ static int term_expec(qg_ctx* c, int dtype, const void* v, int n, const qg_term*
   double* d_part = (double*)(buf + tb);
   double* d_out = (double*)(buf + tb + pb);
   hipError_t err = hipMemcpyAsync(d_t, h.data(), sizeof(PTerm) * nt, hipMemcpyHostToDevice, c->stream);
-  if (err == hipSuccess) err = simple_term_expec(dtype, v, n, d_t, nt, d_part, d_out, c->stream);
+  if (err == hipSuccess) err = simple_term_expec(dtype, v, n, d_t, nt, d_part, d_out, c->stream, h.data());
   if (err == hipSuccess) err = hipMemcpyAsync(e, d_out, sizeof(double) * nt, hipMemcpyDeviceToHost, c->stream);
   if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
   (void)hipFree(buf);
@@ -1610,6 +1610,7 @@ static int mcclean_shift_run(qg_plan* p, const int32_t* axes, const double* angl
     for (int i = 0; i < L; ++i)
       if ((i + 1) % p->ckpt == 0 || i == L - 1) layer_end[i] = wsp(p, p->off_ckpt + (size_t)(m++) * p->S);
   }
+  float total_ms = p->last_ms;  // the unshifted run's
   const void* start0 = ini_state ? wsp(p, p->off_prep) : nullptr;
   // the adjoint's two lambda buffers are free in a forward-only run
   void* buf[2] = {wsp(p, p->off_lam), wsp(p, p->off_lam + p->S)};
@@ -1623,11 +1624,16 @@ static int mcclean_shift_run(qg_plan* p, const int32_t* axes, const double* angl
     int nd = 0;
     void* fin = mcclean_forward_suffix(p, axes, ang.data(), j + 1, j >= 0 ? layer_end[j] : start0, buf, nd);
     ang[k] = angles[k];
-    CHK(run_launches(p, nd));
+    CHK(run_launches(p, nd));  // records c->ev0 before its launches
     HIPCHK(on_shift(s, (const void*)fin));
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
     // the next shift rewrites the pinned descriptor table this one uploads from
     HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    total_ms += ms;
   }
+  p->last_ms = total_ms;  // device time of the unshifted and every shifted circuit
   return QG_OK;
 }
 
@@ -1662,7 +1668,7 @@ extern "C" int qg_mcclean_shift_term_prob(qg_plan* p, const int32_t* axes, const
   CHK(mcclean_shift_run(p, axes, angles, ini_state, nterms ? nshift : 0, shift_param, shift_delta,
                         [&](int s, const void* fin) {
                           return simple_term_expec(p->dtype, fin, p->n, d_t, nt, d_part, d_res + (size_t)s * nt,
-                                                   p->ctx->stream);
+                                                   p->ctx->stream, h.data());
                         }));
   if (nshift == 0 || nterms == 0) return QG_OK;
   std::vector<double> res((size_t)nt * nshift);

@@ -183,21 +183,86 @@ __global__ void __launch_bounds__(256) k_term_expec(const typename V2<S>::T* __r
   }
 }
 
-// out[k] = sum of part[k * np + 0 .. np), one thread per term, in order.
-__global__ void k_sum_terms(const double* __restrict__ part, int np, int nt, double* __restrict__ out) {
-  const int k = threadIdx.x;
-  if (blockIdx.x != 0 || k >= nt) return;
-  double s = 0.0;
-  for (int i = 0; i < np; ++i) s += part[(size_t)k * np + i];
-  out[k] = s;
+// The same sums for diagonal terms only (Z / ZZ, and the identity, kind 3):
+// each term is a bit mask held in registers, <P_k> sums |v_x|^2 (-1)^popc(x & m_k).
+// Two amplitudes per thread and iteration for memory-level parallelism.
+template <class S>
+__global__ void __launch_bounds__(256) k_term_expec_diag(const typename V2<S>::T* __restrict__ v, uint32_t N,
+                                                         const PTerm* __restrict__ t, int nt,
+                                                         double* __restrict__ part) {
+  __shared__ double red[4][kTermsPerLaunch];
+  uint32_t m[kTermsPerLaunch];
+  double acc[kTermsPerLaunch];
+#pragma unroll
+  for (int k = 0; k < kTermsPerLaunch; ++k) {
+    m[k] = 0u;
+    if (k < nt && t[k].kind == 0) m[k] = (1u << t[k].a) | (t[k].b >= 0 ? (1u << t[k].b) : 0u);
+    acc[k] = 0.0;
+  }
+  const uint32_t stride = gridDim.x * blockDim.x;
+  uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  for (; x + stride < N; x += 2 * stride) {
+    const typename V2<S>::T a0 = v[x], a1 = v[x + stride];
+    const double p0 = (double)a0.x * a0.x + (double)a0.y * a0.y;
+    const double p1 = (double)a1.x * a1.x + (double)a1.y * a1.y;
+    const uint32_t x1 = x + stride;
+#pragma unroll
+    for (int k = 0; k < kTermsPerLaunch; ++k) {
+      acc[k] += (__popc(x & m[k]) & 1u) ? -p0 : p0;
+      acc[k] += (__popc(x1 & m[k]) & 1u) ? -p1 : p1;
+    }
+  }
+  if (x < N) {
+    const typename V2<S>::T a0 = v[x];
+    const double p0 = (double)a0.x * a0.x + (double)a0.y * a0.y;
+#pragma unroll
+    for (int k = 0; k < kTermsPerLaunch; ++k) acc[k] += (__popc(x & m[k]) & 1u) ? -p0 : p0;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < kTermsPerLaunch; ++k) {
+    double s = acc[k];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) red[wave][k] = s;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < nt) {
+    const int k = threadIdx.x;
+    part[(size_t)k * gridDim.x + blockIdx.x] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+  }
 }
 
-__global__ void k_sum(const double* __restrict__ part, int np, double* __restrict__ out) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    double s = 0.0;
-    for (int i = 0; i < np; ++i) s += part[i];
-    *out = s;
+// out[k] = sum of part[k * np + 0 .. np): one 256-thread workgroup per term,
+// coalesced strided partial sums, then a fixed-order tree.
+__global__ void __launch_bounds__(256) k_sum_terms(const double* __restrict__ part, int np, int nt,
+                                                   double* __restrict__ out) {
+  __shared__ double red[256];
+  const int k = blockIdx.x;
+  if (k >= nt) return;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < np; i += 256) s += part[(size_t)k * np + i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
   }
+  if (threadIdx.x == 0) out[k] = red[0];
+}
+
+// *out = sum of part[0, np): one 256-thread workgroup, fixed-order tree.
+__global__ void __launch_bounds__(256) k_sum(const double* __restrict__ part, int np, double* __restrict__ out) {
+  __shared__ double red[256];
+  if (blockIdx.x != 0) return;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < np; i += 256) s += part[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
 }
 
 // amp(x) = tab[popcount(x)]  (product initial states)
@@ -381,22 +446,29 @@ hipError_t simple_dot(int dtype, const void* a, const void* b, int n, int mode, 
   const int blocks = (int)((N + 255) / 256) < kDotBlocks ? (int)((N + 255) / 256) : kDotBlocks;
   QG_DT_SWITCH(dtype, hipLaunchKernelGGL(k_dot<S>, dim3(blocks), dim3(256), 0, st, (const typename V2<S>::T*)a,
                                          (const typename V2<S>::T*)b, N, mode, part));
-  hipLaunchKernelGGL(k_sum, dim3(1), dim3(1), 0, st, (const double*)part, blocks, out);
+  hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, st, (const double*)part, blocks, out);
   return hipGetLastError();
 }
 
 // out[k] = <P_k> for the nt terms t (device arrays); part needs
-// kTermBlocks * kTermsPerLaunch doubles.
+// kTermBlocks * kTermsPerLaunch doubles.  h_t: a host copy of t, or NULL;
+// with it, launches of diagonal terms only take the mask kernel.
 hipError_t simple_term_expec(int dtype, const void* v, int n, const PTerm* t, int nt, double* part, double* out,
-                             hipStream_t st) {
+                             hipStream_t st, const PTerm* h_t) {
   if (n < 1 || n > kSimpleMaxN) return hipErrorInvalidValue;
   const uint32_t N = (uint32_t)(1ull << n);
   const int blocks = (int)((N + 255) / 256) < kTermBlocks ? (int)((N + 255) / 256) : kTermBlocks;
   for (int k0 = 0; k0 < nt; k0 += kTermsPerLaunch) {
     const int m = nt - k0 < kTermsPerLaunch ? nt - k0 : kTermsPerLaunch;
-    QG_DT_SWITCH(dtype, hipLaunchKernelGGL(k_term_expec<S>, dim3(blocks), dim3(256), 0, st,
-                                           (const typename V2<S>::T*)v, N, t + k0, m, part));
-    hipLaunchKernelGGL(k_sum_terms, dim3(1), dim3(kTermsPerLaunch), 0, st, (const double*)part, blocks, m, out + k0);
+    bool diag = h_t != nullptr;
+    for (int k = 0; diag && k < m; ++k) diag = h_t[k0 + k].kind == 0 || h_t[k0 + k].kind == 3;
+    if (diag)
+      QG_DT_SWITCH(dtype, hipLaunchKernelGGL(k_term_expec_diag<S>, dim3(blocks), dim3(256), 0, st,
+                                             (const typename V2<S>::T*)v, N, t + k0, m, part));
+    else
+      QG_DT_SWITCH(dtype, hipLaunchKernelGGL(k_term_expec<S>, dim3(blocks), dim3(256), 0, st,
+                                             (const typename V2<S>::T*)v, N, t + k0, m, part));
+    hipLaunchKernelGGL(k_sum_terms, dim3(m), dim3(256), 0, st, (const double*)part, blocks, m, out + k0);
   }
   return hipGetLastError();
 }

@@ -102,6 +102,12 @@ struct Stor<float> {
     const f2v v = *(const gf2v*)((const char*)p + off);
     return {(T)v.x, (T)v.y};
   }
+  // non-temporal: a read-once stream that should not displace cached data
+  template <class T>
+  static __device__ __forceinline__ Cx<T> ld_o_nt(const void* p, uint32_t off) {
+    const f2v v = __builtin_nontemporal_load((const gf2v*)((const char*)p + off));
+    return {(T)v.x, (T)v.y};
+  }
   template <class T>
   static __device__ __forceinline__ void st_o(void* p, uint32_t off, Cx<T> v) {
     f2v w;
@@ -177,6 +183,11 @@ struct Stor<double> {
   template <class T>
   static __device__ __forceinline__ Cx<T> ld_o(const void* p, uint32_t off) {
     const d2v v = *(const gd2v*)((const char*)p + off);
+    return {(T)v.x, (T)v.y};
+  }
+  template <class T>
+  static __device__ __forceinline__ Cx<T> ld_o_nt(const void* p, uint32_t off) {
+    const d2v v = __builtin_nontemporal_load((const gd2v*)((const char*)p + off));
     return {(T)v.x, (T)v.y};
   }
   template <class T>
@@ -773,7 +784,9 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
     if (load_a) {
 #pragma unroll
       for (int a = 0; a < A; ++a)
-        ra[a] = BIG ? Stor<G>::template ld_big<G>(d->a_in, adr[a]) : Stor<G>::template ld_o<G>(d->a_in, adr[a]);
+        ra[a] = BIG ? Stor<G>::template ld_big<G>(d->a_in, adr[a])
+                    : (TWO && !POST) ? Stor<G>::template ld_o_nt<G>(d->a_in, adr[a])
+                                     : Stor<G>::template ld_o<G>(d->a_in, adr[a]);
     } else {
 #pragma unroll
       for (int a = 0; a < A; ++a) ra[a] = {(G)0, (G)0};

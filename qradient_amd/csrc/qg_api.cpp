@@ -819,7 +819,9 @@ static hipError_t launch_all(qg_plan* p) {
   return hipSuccess;
 }
 
-static int run_launches(qg_plan* p, int ndesc) {
+// Prepares and uploads descriptors [first_desc, ndesc) and launches the
+// plan's launch list (whose descriptors lie in that range).
+static int run_launches(qg_plan* p, int ndesc, int first_desc = 0) {
   qg_ctx* c = p->ctx;
   p->h_cterms.clear();
   std::vector<std::pair<int, size_t>> cterm_at;  // (descriptor, offset of its split terms)
@@ -856,7 +858,8 @@ static int run_launches(qg_plan* p, int ndesc) {
                           hipMemcpyHostToDevice, c->stream));
   }
   for (const auto& e : cterm_at) p->h_desc[e.first].cterms = p->d_cterms + e.second;
-  HIPCHK(hipMemcpyAsync(p->d_desc, p->h_desc, sizeof(PassDesc) * ndesc, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(p->d_desc + first_desc, p->h_desc + first_desc, sizeof(PassDesc) * (ndesc - first_desc),
+                        hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   if (p->use_graph && !p->profile) {
     std::vector<int64_t> sig;
@@ -1615,22 +1618,49 @@ static int mcclean_shift_run(qg_plan* p, const int32_t* axes, const double* angl
   // the adjoint's two lambda buffers are free in a forward-only run
   void* buf[2] = {wsp(p, p->off_lam), wsp(p, p->off_lam + p->S)};
   std::vector<double> ang(angles, angles + (size_t)L * n);
-  for (int s = 0; s < nshift; ++s) {
-    const int k = shift_param[s], i = k / n;
+  // Two halves of the descriptor table (a suffix has at most P * L passes):
+  // the host builds shift s + 1 while the device runs shift s, and waits for
+  // shift s - 1, the last user of a half, before rewriting it.
+  const int half = P * L;
+  struct Ev {
+    hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
+    ~Ev() {
+      for (hipEvent_t x : e)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } ev;
+  for (int k = 0; k < 4; ++k) HIPCHK(hipEventCreate(&ev.e[k]));  // done[0], done[1], start, end
+  const bool graph = p->use_graph;
+  p->use_graph = false;  // suffixes differ in length: no graph replay here
+  int rc = QG_OK;
+  hipError_t he = hipSuccess;
+  for (int s = 0; s < nshift && rc == QG_OK && he == hipSuccess; ++s) {
+    const int k = shift_param[s], i = k / n, h = s & 1;
+    if (s >= 2) he = hipEventSynchronize(ev.e[h]);
+    if (he != hipSuccess) break;
     int j = i - 1;
     while (j >= 0 && !layer_end[j]) --j;
     ang[k] = angles[k] + shift_delta[s];
     p->launches.clear();
-    int nd = 0;
+    int nd = h * half;
     void* fin = mcclean_forward_suffix(p, axes, ang.data(), j + 1, j >= 0 ? layer_end[j] : start0, buf, nd);
     ang[k] = angles[k];
-    CHK(run_launches(p, nd));  // records c->ev0 before its launches
-    HIPCHK(on_shift(s, (const void*)fin));
-    HIPCHK(hipEventRecord(c->ev1, c->stream));
-    // the next shift rewrites the pinned descriptor table this one uploads from
-    HIPCHK(hipStreamSynchronize(c->stream));
+    if (s == 0) he = hipEventRecord(ev.e[2], c->stream);
+    if (he != hipSuccess) break;
+    rc = run_launches(p, nd, h * half);
+    if (rc != QG_OK) break;
+    he = on_shift(s, (const void*)fin);
+    if (he == hipSuccess) he = hipEventRecord(ev.e[h], c->stream);
+  }
+  p->use_graph = graph;
+  if (rc == QG_OK && he == hipSuccess && nshift > 0) he = hipEventRecord(ev.e[3], c->stream);
+  const hipError_t hs = hipStreamSynchronize(c->stream);
+  if (rc != QG_OK) return rc;
+  if (he == hipSuccess) he = hs;
+  if (he != hipSuccess) return fail(QG_ERR_HIP, std::string("shifted circuits: ") + hipGetErrorString(he));
+  if (nshift > 0) {
     float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    HIPCHK(hipEventElapsedTime(&ms, ev.e[2], ev.e[3]));
     total_ms += ms;
   }
   p->last_ms = total_ms;  // device time of the unshifted and every shifted circuit

@@ -16,6 +16,13 @@ Files (numpy .npz, no pickled objects):
   qaoa_n8_p3_reg3.npz        QAOA 8 qubits, p = 3, a seeded 3-regular MaxCut graph
   qaoa_reg3_n20_b512.npz     the batched QAOA workload: 512 seeded 3-regular graphs on
                              20 vertices (edges a < b), betas and gammas in [0, pi)
+  sampled_n5.npz             the sampled gradients (mc_clean.py:80-350, qaoa.py:83-158) of a
+                             McClean 5-qubit, 2-layer circuit (positive ZZ-chain weights) and a
+                             QAOA 5-qubit, p = 2 MaxCut circuit, each from numpy's global
+                             RandomState seeded 31 just before the call: E and grad of
+                             grad_run_with_component_sampling, sample_grad(_with_component_
+                             sampling) and sample_grad_dense(_with_component_sampling) with 40
+                             shots, and Qaoa.sample_grad_dense with 40 shots
 """
 import os
 import sys
@@ -90,9 +97,42 @@ def qaoa_batch_workload():
     np.savez(os.path.join(HERE, 'qaoa_reg3_n20_b512.npz'), edges=edges, betas=betas, gammas=gammas)
 
 
+def sampled_n5():
+    from oracle import reference_algorithm as ra
+    n, L, shots, seed = 5, 2, 40, 31
+    rs = np.random.RandomState(8)
+    axes, angles = sv.mcclean_random_params(L, n, rs)
+    obs = sv.zz_chain_observable(n, np.abs(rs.standard_normal(n - 1)) + 0.1)
+    res = {'axes': axes.astype(np.int32), 'angles': angles}
+    res.update(obs_arrays(obs, n))
+
+    def seeded(key, fn):
+        np.random.seed(seed)
+        E, g = fn()
+        res['E_' + key], res['grad_' + key] = np.float64(E), g
+
+    def component_grad_run():
+        dist = sv.weight_distribution(obs)
+        c = np.random.choice(np.arange(dist.size), p=dist)
+        return sv.mcclean_grad_run_component(n, axes, angles, obs, c)
+
+    seeded('component', component_grad_run)
+    seeded('shift', lambda: sv.mcclean_sample_grad(n, axes, angles, obs, shots))
+    seeded('shift_component', lambda: sv.mcclean_sample_grad(n, axes, angles, obs, shots, component_sampling=True))
+    seeded('dense', lambda: ra.mcclean_sample_grad_dense(n, obs, axes, angles, shots))
+    seeded('dense_component', lambda: ra.mcclean_sample_grad_dense(n, obs, axes, angles, shots,
+                                                                    component_sampling=True))
+    edges = np.array([(0, 1), (1, 2), (2, 3), (3, 4), (0, 4), (1, 3)], dtype=np.int32)
+    betas, gammas = np.pi * rs.rand(2), np.pi * rs.rand(2)
+    res.update(edges=edges, betas=betas, gammas=gammas)
+    seeded('qaoa_dense', lambda: ra.qaoa_sample_grad_dense(n, sv.maxcut_observable(n, edges), betas, gammas, shots))
+    np.savez(os.path.join(HERE, 'sampled_n5.npz'), **res)
+
+
 if __name__ == '__main__':
     mcclean_cfg1()
     mcclean_n14()
     qaoa_small()
     qaoa_batch_workload()
+    sampled_n5()
     print('fixtures written to', HERE)

@@ -62,3 +62,39 @@ def test_qaoa_batch_workload_fixture():
     for e in d['edges']:
         assert np.all(e[:, 0] < e[:, 1]) and len({tuple(x) for x in e}) == 30
         assert np.array_equal(np.bincount(e.ravel(), minlength=20), np.full(20, 3))
+
+
+def _sampled_inputs():
+    d = fx.load('sampled_n5.npz')
+    return d, fx.obs_from(d)
+
+
+def test_sampled_gradients_fixture():
+    """The oracle's seeded sampled gradients (mc_clean.py:80-350, qaoa.py:83-158)
+    recomputed from the same seed equal the frozen values."""
+    from oracle import reference_algorithm as ra
+    d, obs = _sampled_inputs()
+    n, shots = 5, 40
+    axes, angles = d['axes'], d['angles']
+
+    def seeded(fn):
+        np.random.seed(31)
+        return fn()
+
+    def component():
+        dist = sv.weight_distribution(obs)
+        c = np.random.choice(np.arange(dist.size), p=dist)
+        return sv.mcclean_grad_run_component(n, axes, angles, obs, c)
+
+    runs = {'component': component,
+            'shift': lambda: sv.mcclean_sample_grad(n, axes, angles, obs, shots),
+            'shift_component': lambda: sv.mcclean_sample_grad(n, axes, angles, obs, shots, component_sampling=True),
+            'dense': lambda: ra.mcclean_sample_grad_dense(n, obs, axes, angles, shots),
+            'dense_component': lambda: ra.mcclean_sample_grad_dense(n, obs, axes, angles, shots,
+                                                                    component_sampling=True),
+            'qaoa_dense': lambda: ra.qaoa_sample_grad_dense(n, fx.maxcut_obs(n, d['edges']), d['betas'],
+                                                            d['gammas'], shots)}
+    for key, fn in runs.items():
+        E, g = seeded(fn)
+        assert abs(E - d['E_' + key]) <= 1e-13, key
+        assert np.max(np.abs(g - d['grad_' + key])) <= 1e-13, key

@@ -53,3 +53,25 @@ def test_qaoa_batch_workload_head(gpu):
         Eo, go = sv.qaoa_grad_run_adjoint(20, obs[b], d['betas'][b], d['gammas'][b])
         assert abs(E[b] - Eo) <= 1e-12 * max(1.0, abs(Eo))
         assert rel_err(g[b], go) <= 1e-12
+
+
+def test_sampled_gradients(gpu):
+    """Seeded sampled gradients of the library equal the frozen oracle values
+    (same random numbers, complex128 probabilities within 1e-12)."""
+    from qradient_amd.circuit_logic import McClean, Qaoa
+    d = fx.load('sampled_n5.npz')
+    obs = fx.obs_from(d)
+    n, L, shots = 5, 2, 40
+    circ = McClean(n, obs, L, axes=d['axes'], angles=d['angles'], use_observable_components=True)
+    runs = {'component': lambda: circ.grad_run_with_component_sampling(),
+            'shift': lambda: circ.sample_grad(shot_num=shots),
+            'shift_component': lambda: circ.sample_grad_with_component_sampling(shot_num=shots),
+            'dense': lambda: circ.sample_grad_dense(shot_num=shots),
+            'dense_component': lambda: circ.sample_grad_dense_with_component_sampling(shot_num=shots),
+            'qaoa_dense': lambda: Qaoa(n, fx.maxcut_obs(n, d['edges']), 2).sample_grad_dense(
+                d['betas'], d['gammas'], shot_num=shots)}
+    for key, fn in runs.items():
+        np.random.seed(31)
+        E, g = fn()
+        assert abs(E - d['E_' + key]) <= 1e-12, key
+        assert np.max(np.abs(g - d['grad_' + key])) <= 1e-12, key

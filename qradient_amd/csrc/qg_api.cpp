@@ -414,7 +414,12 @@ static size_t lds_bytes_of(int dtype, int K, int threads) {  // the larger (adjo
 // Workgroups per pass launch: as many as are resident at once on the device
 // (occupancy of the kernel variant, by its registers and LDS), rounded down to
 // a power of two and at most one per tile.
-static int grid_for_pass(const qg_plan* p, const Geo& g, bool two, bool post, int phase) {
+constexpr int kBatchWgPerCu = 16;  // workgroups per CU of a batched QAOA pass launch
+constexpr int kWgOversub = 4;      // forward pass grids: this many times the resident workgroups
+
+// Workgroups of this pass variant resident per CU (occupancy API, or an LDS
+// estimate where the query fails).
+static int pass_occupancy(const qg_plan* p, const Geo& g, bool two, bool post, int phase) {
   int occ = 0;
   if (launch_pass(kdtype(p), g.R, two, phase, post, nullptr, g.K, 0, 1, g.threads, p->n, nullptr, &occ, 0) !=
           hipSuccess ||
@@ -423,8 +428,16 @@ static int grid_for_pass(const qg_plan* p, const Geo& g, bool two, bool post, in
     occ = (int)std::max<size_t>(1, (160 * 1024) / lds);
     occ = std::min(occ, std::max(1, 512 / g.threads));
   }
+  return occ;
+}
+
+static int grid_for_pass(const qg_plan* p, const Geo& g, bool two, bool post, int phase) {
+  const int occ = pass_occupancy(p, g, two, post, phase);
   const uint64_t ntiles = 1ull << (p->n - g.K);
-  uint64_t target = (uint64_t)p->ctx->num_cu * occ;
+  // forward passes: more workgroups than are resident spread the tiles more
+  // evenly; adjoint passes keep one resident wave of long-lived workgroups
+  // (their reduction accumulators last across the workgroup's tiles)
+  uint64_t target = (uint64_t)p->ctx->num_cu * occ * (two ? 1 : kWgOversub);
   uint64_t grid = 1;
   while (grid * 2 <= target && grid * 2 <= ntiles) grid *= 2;
   return (int)grid;
@@ -1988,8 +2001,10 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
       l.threads = G.threads;
       l.grid = grid_for_pass(p, G, two, tf, phase);
       // a batched launch fills the device across circuits: fewer workgroups
-      // per circuit, each running more tiles (amortises its prologue)
-      while (l.grid > 1 && (uint64_t)l.grid * nb > 2ull * (uint64_t)p->ctx->num_cu) l.grid >>= 1;
+      // per circuit, each running more tiles (amortises its prologue); up to
+      // kBatchWgPerCu per CU in all, beyond the resident count, so that the
+      // circuits' tiles spread evenly over the CUs
+      while (l.grid > 1 && (uint64_t)l.grid * nb > (uint64_t)kBatchWgPerCu * p->ctx->num_cu) l.grid >>= 1;
       l.phase = phase;
       l.two = two;
       l.kind = 0;

@@ -184,44 +184,48 @@ __global__ void __launch_bounds__(256) k_term_expec(const typename V2<S>::T* __r
 }
 
 // The same sums for diagonal terms only (Z / ZZ, and the identity, kind 3):
-// each term is a bit mask held in registers, <P_k> sums |v_x|^2 (-1)^popc(x & m_k).
-// Two amplitudes per thread and iteration for memory-level parallelism.
+// <P_k> sums |v_x|^2 (-1)^popc(x & m_k).  The grid stride is a power of two
+// (2^sb amplitudes), so x = g + j 2^sb with g the thread's global index and j
+// the iteration: the sign splits into (-1)^popc(g & m_k), fixed per thread,
+// and (-1)^popc(j & (m_k >> sb)), the same for the whole grid at iteration j.
+// Each thread sums (-1)^popc(j & mh_k) |v_x|^2 over its iterations (one FMA
+// per term and amplitude) and applies its own sign at the end.
 template <class S>
 __global__ void __launch_bounds__(256) k_term_expec_diag(const typename V2<S>::T* __restrict__ v, uint32_t N,
-                                                         const PTerm* __restrict__ t, int nt,
+                                                         const PTerm* __restrict__ t, int nt, int sb,
                                                          double* __restrict__ part) {
   __shared__ double red[4][kTermsPerLaunch];
-  uint32_t m[kTermsPerLaunch];
-  double acc[kTermsPerLaunch];
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t mh[kTermsPerLaunch];
+  double acc[kTermsPerLaunch], sl[kTermsPerLaunch];
 #pragma unroll
   for (int k = 0; k < kTermsPerLaunch; ++k) {
-    m[k] = 0u;
-    if (k < nt && t[k].kind == 0) m[k] = (1u << t[k].a) | (t[k].b >= 0 ? (1u << t[k].b) : 0u);
+    uint32_t m = 0u;
+    if (k < nt && t[k].kind == 0) m = (1u << t[k].a) | (t[k].b >= 0 ? (1u << t[k].b) : 0u);
+    mh[k] = sb < 32 ? m >> sb : 0u;
+    sl[k] = (__popc(g & m) & 1u) ? -1.0 : 1.0;
     acc[k] = 0.0;
   }
-  const uint32_t stride = gridDim.x * blockDim.x;
-  uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
-  for (; x + stride < N; x += 2 * stride) {
-    const typename V2<S>::T a0 = v[x], a1 = v[x + stride];
+  const uint32_t iters = N >> sb;  // N and the stride are powers of two
+  for (uint32_t j = 0; j < iters; j += 2) {
+    const typename V2<S>::T a0 = v[g + (j << sb)];
     const double p0 = (double)a0.x * a0.x + (double)a0.y * a0.y;
-    const double p1 = (double)a1.x * a1.x + (double)a1.y * a1.y;
-    const uint32_t x1 = x + stride;
+    double p1 = 0.0;
+    if (j + 1 < iters) {
+      const typename V2<S>::T a1 = v[g + ((j + 1) << sb)];
+      p1 = (double)a1.x * a1.x + (double)a1.y * a1.y;
+    }
 #pragma unroll
     for (int k = 0; k < kTermsPerLaunch; ++k) {
-      acc[k] += (__popc(x & m[k]) & 1u) ? -p0 : p0;
-      acc[k] += (__popc(x1 & m[k]) & 1u) ? -p1 : p1;
+      const double s0 = (__popc(j & mh[k]) & 1u) ? -1.0 : 1.0;
+      const double s1 = (__popc((j + 1) & mh[k]) & 1u) ? -1.0 : 1.0;
+      acc[k] = __builtin_fma(s1, p1, __builtin_fma(s0, p0, acc[k]));
     }
-  }
-  if (x < N) {
-    const typename V2<S>::T a0 = v[x];
-    const double p0 = (double)a0.x * a0.x + (double)a0.y * a0.y;
-#pragma unroll
-    for (int k = 0; k < kTermsPerLaunch; ++k) acc[k] += (__popc(x & m[k]) & 1u) ? -p0 : p0;
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < kTermsPerLaunch; ++k) {
-    double s = acc[k];
+    double s = sl[k] * acc[k];
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     if (lane == 0) red[wave][k] = s;
   }
@@ -462,12 +466,16 @@ hipError_t simple_term_expec(int dtype, const void* v, int n, const PTerm* t, in
     const int m = nt - k0 < kTermsPerLaunch ? nt - k0 : kTermsPerLaunch;
     bool diag = h_t != nullptr;
     for (int k = 0; diag && k < m; ++k) diag = h_t[k0 + k].kind == 0 || h_t[k0 + k].kind == 3;
-    if (diag)
+    // the diagonal kernel needs one thread per amplitude or fewer: N >= 256
+    if (diag && N >= 256) {
+      int sb = 0;
+      while ((1u << sb) < (uint32_t)blocks * 256u) ++sb;
       QG_DT_SWITCH(dtype, hipLaunchKernelGGL(k_term_expec_diag<S>, dim3(blocks), dim3(256), 0, st,
-                                             (const typename V2<S>::T*)v, N, t + k0, m, part));
-    else
+                                             (const typename V2<S>::T*)v, N, t + k0, m, sb, part));
+    } else {
       QG_DT_SWITCH(dtype, hipLaunchKernelGGL(k_term_expec<S>, dim3(blocks), dim3(256), 0, st,
                                              (const typename V2<S>::T*)v, N, t + k0, m, part));
+    }
     hipLaunchKernelGGL(k_sum_terms, dim3(m), dim3(256), 0, st, (const double*)part, blocks, m, out + k0);
   }
   return hipGetLastError();

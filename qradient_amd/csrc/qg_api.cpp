@@ -322,6 +322,8 @@ struct qg_plan {
   bool fused = true;       // n <= 13: whole circuit per workgroup (qg_fused.hip)
   char* h_stage = nullptr;  // pinned staging of QAOA term and phase tables
   size_t stage_cap = 0;
+  DTerm* h_cstage = nullptr;  // pinned staging of split terms (streamed launches)
+  size_t cstage_cap = 0;
   char* h_arena = nullptr;  // per-call tables, pinned host
   char* d_arena = nullptr;  // and their device copy
   size_t arena_cap = 0;
@@ -936,6 +938,85 @@ static int run_launches(qg_plan* p, int ndesc, int first_desc = 0) {
   return QG_OK;
 }
 
+// Batched QAOA runs carry thousands of descriptors.  This prepares, uploads
+// and launches `group` launches at a time (the preparation of run_launches),
+// so the device starts on the first passes while the host prepares the
+// rest.  Graph replay and profiled runs take run_launches.
+static int run_launches_streamed(qg_plan* p, int ndesc, int group) {
+  if (p->use_graph || p->profile) return run_launches(p, ndesc);
+  qg_ctx* c = p->ctx;
+  const uint32_t split_flags = F_PHASE_PRE | F_PHASE_POST | F_DRED_PRE | F_DRED_POST | F_OBS_DIAG;
+  // a descriptor splits at most its ndterms terms
+  size_t bound = 0;
+  for (const Launch& l : p->launches)
+    if (l.kind == 0 && (p->h_desc[l.desc].flags & split_flags))
+      for (int b = 0; b < l.batch; ++b) bound += (size_t)std::max(0, p->h_desc[l.desc + b].ndterms);
+  if (bound > p->cterm_cap) {
+    HIPCHK(hipStreamSynchronize(c->stream));  // the old table may still be read
+    (void)hipFree(p->d_cterms);
+    p->d_cterms = nullptr;
+    p->cterm_cap = 0;
+    HIPCHK(hipMalloc(&p->d_cterms, sizeof(DTerm) * bound));
+    p->cterm_cap = bound;
+  }
+  if (bound > p->cstage_cap) {
+    if (p->h_cstage) (void)hipHostFree(p->h_cstage);
+    p->h_cstage = nullptr;
+    p->cstage_cap = 0;
+    HIPCHK(hipHostMalloc(&p->h_cstage, sizeof(DTerm) * bound, hipHostMallocDefault));
+    p->cstage_cap = bound;
+  }
+  p->h_cterms.clear();
+  p->h_cterms.reserve(bound);
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  const size_t nl = p->launches.size();
+  for (size_t g0 = 0; g0 < nl; g0 += (size_t)group) {
+    const size_t g1 = std::min(nl, g0 + (size_t)group);
+    const size_t ct0 = p->h_cterms.size();
+    int d0 = ndesc, d1 = 0;
+    for (size_t k = g0; k < g1; ++k) {
+      Launch& l = p->launches[k];
+      if (l.kind != 0) continue;
+      if (l.grid > p->gmax) return fail(QG_ERR_HIP, "internal: pass grid exceeds the partial-row stride");
+      const PassDesc& d = p->h_desc[l.desc];
+      if (!(d.flags & (F_PHASE_PRE | F_PHASE_POST))) l.phase = 0;
+      l.post = l.two && ((d.flags & (F_DRED_POST | F_PHASE_POST | F_OBS_DIAG | F_STORE_A)) != 0 ||
+                         !(d.flags & F_LOAD_A) || ((d.flags & F_PHASE_PRE) && !(d.flags & F_DRED_PRE)));
+      for (int b = 0; b < l.batch; ++b) desc_slots(p->h_desc[l.desc + b], l.R, l.two, l.two && !l.post);
+      if (d.flags & split_flags)
+        for (int b = 0; b < l.batch; ++b) {
+          const size_t off = split_terms(p, p->h_desc[l.desc + b], l.R, p->h_cterms);
+          p->h_desc[l.desc + b].cterms = p->d_cterms + off;
+        }
+      l.tab = ((d.flags & F_INIT_TAB) ? (int)sizeof(double) * (p->n + 1) : 0) +
+              ((l.phase == 1 && (d.flags & (F_PHASE_PRE | F_PHASE_POST))) ? (int)sizeof(double) * 2 * (2 * d.lut_off + 1)
+                                                                           : 0);
+      d0 = std::min(d0, l.desc);
+      d1 = std::max(d1, l.desc + l.batch);
+    }
+    const size_t ct1 = p->h_cterms.size();
+    if (ct1 > bound) return fail(QG_ERR_HIP, "internal: split terms exceed their bound");
+    if (ct1 > ct0) {
+      std::memcpy(p->h_cstage + ct0, p->h_cterms.data() + ct0, sizeof(DTerm) * (ct1 - ct0));
+      HIPCHK(hipMemcpyAsync(p->d_cterms + ct0, p->h_cstage + ct0, sizeof(DTerm) * (ct1 - ct0),
+                            hipMemcpyHostToDevice, c->stream));
+    }
+    if (d1 > d0)
+      HIPCHK(hipMemcpyAsync(p->d_desc + d0, p->h_desc + d0, sizeof(PassDesc) * (d1 - d0), hipMemcpyHostToDevice,
+                            c->stream));
+    for (size_t k = g0; k < g1; ++k) {
+      const Launch& l = p->launches[k];
+      if (l.kind == 0) {
+        HIPCHK(launch_pass(kdtype(p), l.R, l.two, l.phase, l.post, p->d_desc + l.desc, l.K, l.grid, l.batch,
+                           l.threads, p->n, c->stream, nullptr, l.tab));
+      } else if (l.kind == 1) {
+        HIPCHK(launch_generic_obs(p));
+      }
+    }
+  }
+  return QG_OK;
+}
+
 static int collect_profile(qg_plan* p) {
   if (!p->profile) return QG_OK;
   const size_t nl = p->launches.size();
@@ -996,6 +1077,9 @@ static void free_small(qg_plan* p) {
   if (p->h_out) (void)hipHostFree(p->h_out);
   if (p->h_arena) (void)hipHostFree(p->h_arena);
   if (p->h_stage) (void)hipHostFree(p->h_stage);
+  if (p->h_cstage) (void)hipHostFree(p->h_cstage);
+  p->h_cstage = nullptr;
+  p->cstage_cap = 0;
   (void)hipFree(p->d_arena);
   (void)hipFree(p->d_fstate);
   p->h_stage = nullptr;
@@ -2124,7 +2208,7 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
       for (int j = 0; j < nb; ++j)
         for (int k = 1; k < per_out; ++k) out_rows[(size_t)j * per_out + k].clear();
     }
-    CHK(run_launches(p, nd));
+    CHK(run_launches_streamed(p, nd, 2));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     CHK(collect_profile(p));
     CHK(upload_finalize(p, out_rows, cnt, scale));

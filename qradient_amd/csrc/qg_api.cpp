@@ -324,6 +324,8 @@ struct qg_plan {
   size_t stage_cap = 0;
   DTerm* h_cstage = nullptr;  // pinned staging of split terms (streamed launches)
   size_t cstage_cap = 0;
+  size_t stream_bound = 0;  // streamed run: split-term capacity
+  size_t stream_next = 0;   // streamed run: first launch not yet flushed
   char* h_arena = nullptr;  // per-call tables, pinned host
   char* d_arena = nullptr;  // and their device copy
   size_t arena_cap = 0;
@@ -938,19 +940,14 @@ static int run_launches(qg_plan* p, int ndesc, int first_desc = 0) {
   return QG_OK;
 }
 
-// Batched QAOA runs carry thousands of descriptors.  This prepares, uploads
-// and launches `group` launches at a time (the preparation of run_launches),
-// so the device starts on the first passes while the host prepares the
-// rest.  Graph replay and profiled runs take run_launches.
-static int run_launches_streamed(qg_plan* p, int ndesc, int group) {
-  if (p->use_graph || p->profile) return run_launches(p, ndesc);
+// Batched QAOA runs carry thousands of descriptors.  A streamed run
+// prepares, uploads and launches the plan's launches in groups while the
+// host is still building later ones, so the device starts on the first
+// passes early.  stream_begin sizes the split-term table for at most `bound`
+// terms; stream_flush launches every launch built so far (from
+// p->stream_next on); descriptors of flushed launches must not change.
+static int stream_begin(qg_plan* p, size_t bound) {
   qg_ctx* c = p->ctx;
-  const uint32_t split_flags = F_PHASE_PRE | F_PHASE_POST | F_DRED_PRE | F_DRED_POST | F_OBS_DIAG;
-  // a descriptor splits at most its ndterms terms
-  size_t bound = 0;
-  for (const Launch& l : p->launches)
-    if (l.kind == 0 && (p->h_desc[l.desc].flags & split_flags))
-      for (int b = 0; b < l.batch; ++b) bound += (size_t)std::max(0, p->h_desc[l.desc + b].ndterms);
   if (bound > p->cterm_cap) {
     HIPCHK(hipStreamSynchronize(c->stream));  // the old table may still be read
     (void)hipFree(p->d_cterms);
@@ -966,54 +963,63 @@ static int run_launches_streamed(qg_plan* p, int ndesc, int group) {
     HIPCHK(hipHostMalloc(&p->h_cstage, sizeof(DTerm) * bound, hipHostMallocDefault));
     p->cstage_cap = bound;
   }
+  p->stream_bound = bound;
+  p->stream_next = 0;
   p->h_cterms.clear();
   p->h_cterms.reserve(bound);
   HIPCHK(hipEventRecord(c->ev0, c->stream));
-  const size_t nl = p->launches.size();
-  for (size_t g0 = 0; g0 < nl; g0 += (size_t)group) {
-    const size_t g1 = std::min(nl, g0 + (size_t)group);
-    const size_t ct0 = p->h_cterms.size();
-    int d0 = ndesc, d1 = 0;
-    for (size_t k = g0; k < g1; ++k) {
-      Launch& l = p->launches[k];
-      if (l.kind != 0) continue;
-      if (l.grid > p->gmax) return fail(QG_ERR_HIP, "internal: pass grid exceeds the partial-row stride");
-      const PassDesc& d = p->h_desc[l.desc];
-      if (!(d.flags & (F_PHASE_PRE | F_PHASE_POST))) l.phase = 0;
-      l.post = l.two && ((d.flags & (F_DRED_POST | F_PHASE_POST | F_OBS_DIAG | F_STORE_A)) != 0 ||
-                         !(d.flags & F_LOAD_A) || ((d.flags & F_PHASE_PRE) && !(d.flags & F_DRED_PRE)));
-      for (int b = 0; b < l.batch; ++b) desc_slots(p->h_desc[l.desc + b], l.R, l.two, l.two && !l.post);
-      if (d.flags & split_flags)
-        for (int b = 0; b < l.batch; ++b) {
-          const size_t off = split_terms(p, p->h_desc[l.desc + b], l.R, p->h_cterms);
-          p->h_desc[l.desc + b].cterms = p->d_cterms + off;
-        }
-      l.tab = ((d.flags & F_INIT_TAB) ? (int)sizeof(double) * (p->n + 1) : 0) +
-              ((l.phase == 1 && (d.flags & (F_PHASE_PRE | F_PHASE_POST))) ? (int)sizeof(double) * 2 * (2 * d.lut_off + 1)
-                                                                           : 0);
-      d0 = std::min(d0, l.desc);
-      d1 = std::max(d1, l.desc + l.batch);
-    }
-    const size_t ct1 = p->h_cterms.size();
-    if (ct1 > bound) return fail(QG_ERR_HIP, "internal: split terms exceed their bound");
-    if (ct1 > ct0) {
-      std::memcpy(p->h_cstage + ct0, p->h_cterms.data() + ct0, sizeof(DTerm) * (ct1 - ct0));
-      HIPCHK(hipMemcpyAsync(p->d_cterms + ct0, p->h_cstage + ct0, sizeof(DTerm) * (ct1 - ct0),
-                            hipMemcpyHostToDevice, c->stream));
-    }
-    if (d1 > d0)
-      HIPCHK(hipMemcpyAsync(p->d_desc + d0, p->h_desc + d0, sizeof(PassDesc) * (d1 - d0), hipMemcpyHostToDevice,
-                            c->stream));
-    for (size_t k = g0; k < g1; ++k) {
-      const Launch& l = p->launches[k];
-      if (l.kind == 0) {
-        HIPCHK(launch_pass(kdtype(p), l.R, l.two, l.phase, l.post, p->d_desc + l.desc, l.K, l.grid, l.batch,
-                           l.threads, p->n, c->stream, nullptr, l.tab));
-      } else if (l.kind == 1) {
-        HIPCHK(launch_generic_obs(p));
+  return QG_OK;
+}
+
+static int stream_flush(qg_plan* p) {
+  qg_ctx* c = p->ctx;
+  const uint32_t split_flags = F_PHASE_PRE | F_PHASE_POST | F_DRED_PRE | F_DRED_POST | F_OBS_DIAG;
+  const size_t g0 = p->stream_next, g1 = p->launches.size();
+  if (g1 <= g0) return QG_OK;
+  const size_t ct0 = p->h_cterms.size();
+  int d0 = 0x7fffffff, d1 = 0;
+  for (size_t k = g0; k < g1; ++k) {
+    Launch& l = p->launches[k];
+    if (l.kind != 0) continue;
+    if (l.grid > p->gmax) return fail(QG_ERR_HIP, "internal: pass grid exceeds the partial-row stride");
+    const PassDesc& d = p->h_desc[l.desc];
+    // the preparation of run_launches
+    if (!(d.flags & (F_PHASE_PRE | F_PHASE_POST))) l.phase = 0;
+    l.post = l.two && ((d.flags & (F_DRED_POST | F_PHASE_POST | F_OBS_DIAG | F_STORE_A)) != 0 ||
+                       !(d.flags & F_LOAD_A) || ((d.flags & F_PHASE_PRE) && !(d.flags & F_DRED_PRE)));
+    for (int b = 0; b < l.batch; ++b) desc_slots(p->h_desc[l.desc + b], l.R, l.two, l.two && !l.post);
+    if (d.flags & split_flags)
+      for (int b = 0; b < l.batch; ++b) {
+        if (p->h_cterms.size() + (size_t)std::max(0, p->h_desc[l.desc + b].ndterms) > p->stream_bound)
+          return fail(QG_ERR_HIP, "internal: split terms exceed their bound");
+        const size_t off = split_terms(p, p->h_desc[l.desc + b], l.R, p->h_cterms);
+        p->h_desc[l.desc + b].cterms = p->d_cterms + off;
       }
+    l.tab = ((d.flags & F_INIT_TAB) ? (int)sizeof(double) * (p->n + 1) : 0) +
+            ((l.phase == 1 && (d.flags & (F_PHASE_PRE | F_PHASE_POST))) ? (int)sizeof(double) * 2 * (2 * d.lut_off + 1)
+                                                                         : 0);
+    d0 = std::min(d0, l.desc);
+    d1 = std::max(d1, l.desc + l.batch);
+  }
+  const size_t ct1 = p->h_cterms.size();
+  if (ct1 > ct0) {
+    std::memcpy(p->h_cstage + ct0, p->h_cterms.data() + ct0, sizeof(DTerm) * (ct1 - ct0));
+    HIPCHK(hipMemcpyAsync(p->d_cterms + ct0, p->h_cstage + ct0, sizeof(DTerm) * (ct1 - ct0), hipMemcpyHostToDevice,
+                          c->stream));
+  }
+  if (d1 > d0)
+    HIPCHK(hipMemcpyAsync(p->d_desc + d0, p->h_desc + d0, sizeof(PassDesc) * (d1 - d0), hipMemcpyHostToDevice,
+                          c->stream));
+  for (size_t k = g0; k < g1; ++k) {
+    const Launch& l = p->launches[k];
+    if (l.kind == 0) {
+      HIPCHK(launch_pass(kdtype(p), l.R, l.two, l.phase, l.post, p->d_desc + l.desc, l.K, l.grid, l.batch,
+                         l.threads, p->n, c->stream, nullptr, l.tab));
+    } else if (l.kind == 1) {
+      HIPCHK(launch_generic_obs(p));
     }
   }
+  p->stream_next = g1;
   return QG_OK;
 }
 
@@ -2114,6 +2120,15 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
     // rotated back.  With one pass per layer the phase opens each pass and
     // the adjoint takes the general form.
     const bool phase_last = P >= 2;
+    // passes go to the device as they are built (graph replay and profiled
+    // runs launch the whole list at the end); each descriptor splits at most
+    // its circuit's terms
+    const bool streamed = !p->use_graph && !p->profile;
+    if (streamed) {
+      size_t per_pass = 0;
+      for (int j = 0; j < nb; ++j) per_pass += (size_t)tcnt[j];
+      CHK(stream_begin(p, per_pass * (size_t)P * L * (grad ? 2 : 1)));
+    }
     // forward
     for (int i = 0; i < L; ++i) {
       for (int g = 0; g < P; ++g) {
@@ -2154,6 +2169,7 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
           }
           d.flags = f;
         }
+        if (streamed && p->launches.size() - p->stream_next >= 2) CHK(stream_flush(p));
       }
     }
     if (grad) {
@@ -2200,6 +2216,7 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
             for (int k = 0; k <= kMaxK; ++k) cnt[row + k] = gr;
             row += kMaxK + 1;
           }
+          if (streamed && p->launches.size() - p->stream_next >= 2) CHK(stream_flush(p));
         }
       }
     }
@@ -2208,7 +2225,11 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
       for (int j = 0; j < nb; ++j)
         for (int k = 1; k < per_out; ++k) out_rows[(size_t)j * per_out + k].clear();
     }
-    CHK(run_launches_streamed(p, nd, 2));
+    if (streamed) {
+      CHK(stream_flush(p));
+    } else {
+      CHK(run_launches(p, nd));
+    }
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     CHK(collect_profile(p));
     CHK(upload_finalize(p, out_rows, cnt, scale));

@@ -64,17 +64,19 @@ def time_steps(circ, steps, warmup, sync, profile):
     t0 = time.perf_counter()
     for _ in range(steps):
         E, g = circ.grad_run()
-        if profile:
-            per.append(pass_times(circ, _native))
     sync()
     dt = time.perf_counter() - t0
     if profile:
+        # every launch of the timed steps carried HIP events on the library's
+        # stream; their times are read now, outside the timed region
+        per.append(pass_times(circ, _native))
         _native.call('qg_plan_set_profiling', circ._plan, 0)
     return dt, E, g, per
 
 
-def roofline(per, state_bytes):
-    """Dominant pass class: reverse two-state passes (read A, read B, write B)."""
+def roofline(per, state_bytes, steps):
+    """Dominant pass class: reverse two-state passes (read A, read B, write B).
+    ``per``: (ms, kinds) of every launch of the ``steps`` timed steps."""
     ms = np.concatenate([p[0] for p in per])
     kinds = np.concatenate([p[1] for p in per])
     alg = {0: 2 * state_bytes, 1: 3 * state_bytes, 2: 3 * state_bytes}
@@ -82,7 +84,7 @@ def roofline(per, state_bytes):
     dom = max(totals, key=totals.get)
     avg_ms = float(ms[kinds == dom].mean())
     achieved = alg[dom] / (avg_ms * 1e-3) / 1e9
-    return dom, avg_ms, achieved, alg[dom], {int(k): float(v) / len(per) for k, v in totals.items()}
+    return dom, avg_ms, achieved, alg[dom], {int(k): float(v) / steps for k, v in totals.items()}
 
 
 def cpu_baseline(axes, angles, obs):
@@ -337,7 +339,7 @@ def main():
         steps = args.steps if dtype == 'complex64' else max(2, args.steps // 2)
         dt, E, g, per = time_steps(circ, steps, args.warmup, sync, profile=True)
         sb = (2 ** N_QUBITS) * (8 if dtype == 'complex64' else 16)
-        res[dtype] = dict(ms=dt / steps * 1e3, E=E, roof=roofline(per, sb), steps=steps)
+        res[dtype] = dict(ms=dt / steps * 1e3, E=E, roof=roofline(per, sb, steps), steps=steps)
         grads[dtype] = g
         circ._destroy_plan()
     c64, c128 = res['complex64'], res['complex128']

@@ -175,11 +175,14 @@ int qg_plan_term_prob(qg_plan* plan, const qg_term* terms, int nterms, double* p
 int qg_plan_expec_val(qg_plan* plan, const qg_term* terms, int nterms, double* E_out);
 /* Device time of the last run in milliseconds (hipEvents around the launches). */
 int qg_plan_last_ms(const qg_plan* plan, double* ms);
-/* Record a HIP event before every kernel launch of later runs (on = 1). */
+/* Record a HIP event before every kernel launch of later runs (on = 1).  Each
+ * call starts a new series of profiled runs; a plan keeps the latest 256. */
 int qg_plan_set_profiling(qg_plan* plan, int on);
-/* Per-launch device times (ms) of the last profiled run and their kind
- * (0 forward pass, 1 reverse two-state pass, 2 forward pass fused with the
- * observable); *count receives the launch count, at most cap are written. */
+/* Per-launch device times (ms) of the profiled runs of the current series,
+ * run after run, and their kind (0 forward pass, 1 reverse two-state pass,
+ * 2 forward pass fused with the observable, 3 another launch); *count
+ * receives the launch count, at most cap are written.  The times are read
+ * from the events here, so profiled runs pay only for the event records. */
 int qg_plan_pass_times(const qg_plan* plan, int cap, float* ms, int* kinds, int* count);
 /* Tile shape of a McClean plan: kmax local bits per pass (4..14; 14 only for
  * QG_C64 plans that keep the per-pass history with FP32 tiles), at least

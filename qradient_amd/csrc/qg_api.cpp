@@ -367,9 +367,13 @@ struct qg_plan {
   float last_ms = 0.f;
   // per-launch profiling: one event before every launch and one after the last
   bool profile = false;
-  std::vector<hipEvent_t> pev;
-  std::vector<float> pass_ms;  // durations of the last profiled run
-  std::vector<int> pass_kind;  // 0 forward, 1 reverse (two-state), 2 forward+observable
+  // one event set per profiled run since profiling was switched on (the
+  // launches' kinds beside it); times are read when they are asked for
+  std::vector<std::vector<hipEvent_t>> prof_ev;
+  std::vector<std::vector<int>> prof_kind;
+  size_t prof_runs = 0;
+  mutable std::vector<float> pass_ms;  // durations of those runs' launches
+  mutable std::vector<int> pass_kind;  // 0 forward, 1 reverse (two-state), 2 forward+observable, 3 other
   int last_state = -1;  // 0: final state in hist, used by get_state
   // the pass launches of a run as a HIP graph, replayed while the launch list
   // (variants, grids, descriptor slots) is unchanged; the kernels read their
@@ -418,6 +422,7 @@ static size_t lds_bytes_of(int dtype, int K, int threads) {  // the larger (adjo
 // Workgroups per pass launch: as many as are resident at once on the device
 // (occupancy of the kernel variant, by its registers and LDS), rounded down to
 // a power of two and at most one per tile.
+constexpr size_t kMaxProfRuns = 256;  // profiled runs whose launch events a plan keeps
 constexpr int kBatchWgPerCu = 16;  // workgroups per CU of a batched QAOA pass launch
 constexpr int kWgOversub = 4;      // forward pass grids: this many times the resident workgroups
 
@@ -500,7 +505,8 @@ extern "C" int qg_plan_destroy(qg_plan* p) {
   if (p->ws_owned && p->ws) (void)hipFree(p->ws);
   free_small(p);
   drop_graph(p);
-  for (hipEvent_t e : p->pev) (void)hipEventDestroy(e);
+  for (auto& set : p->prof_ev)
+    for (hipEvent_t e : set) (void)hipEventDestroy(e);
   delete p;
   return QG_OK;
 }
@@ -919,16 +925,20 @@ static int run_launches(qg_plan* p, int ndesc, int first_desc = 0) {
     }
   }
   const size_t nl = p->launches.size();
+  std::vector<hipEvent_t>* pev = nullptr;
   if (p->profile) {
-    while (p->pev.size() < nl + 1) {
+    if (p->prof_runs >= kMaxProfRuns) p->prof_runs = 0;  // start over: only the latest runs are kept
+    if (p->prof_ev.size() <= p->prof_runs) p->prof_ev.resize(p->prof_runs + 1);
+    pev = &p->prof_ev[p->prof_runs];
+    while (pev->size() < nl + 1) {
       hipEvent_t e;
       HIPCHK(hipEventCreate(&e));
-      p->pev.push_back(e);
+      pev->push_back(e);
     }
   }
   for (size_t k = 0; k < nl; ++k) {
     const Launch& l = p->launches[k];
-    if (p->profile) HIPCHK(hipEventRecord(p->pev[k], c->stream));
+    if (pev) HIPCHK(hipEventRecord((*pev)[k], c->stream));
     if (l.kind == 0) {
       HIPCHK(launch_pass(kdtype(p), l.R, l.two, l.phase, l.post, p->d_desc + l.desc, l.K, l.grid, l.batch,
                          l.threads, p->n, c->stream, nullptr, l.tab));
@@ -936,7 +946,7 @@ static int run_launches(qg_plan* p, int ndesc, int first_desc = 0) {
       HIPCHK(launch_generic_obs(p));
     }
   }
-  if (p->profile) HIPCHK(hipEventRecord(p->pev[nl], c->stream));
+  if (pev) HIPCHK(hipEventRecord((*pev)[nl], c->stream));
   return QG_OK;
 }
 
@@ -1023,32 +1033,54 @@ static int stream_flush(qg_plan* p) {
   return QG_OK;
 }
 
+// Notes the kinds of a profiled run's launches (its events were recorded by
+// run_launches); the times are read in qg_plan_pass_times, so a profiled run
+// costs its event records only.
 static int collect_profile(qg_plan* p) {
   if (!p->profile) return QG_OK;
   const size_t nl = p->launches.size();
-  p->pass_ms.assign(nl, 0.f);
-  p->pass_kind.assign(nl, 0);
-  HIPCHK(hipEventSynchronize(p->pev[nl]));
+  if (p->prof_kind.size() <= p->prof_runs) p->prof_kind.resize(p->prof_runs + 1);
+  std::vector<int>& kinds = p->prof_kind[p->prof_runs];
+  kinds.assign(nl, 0);
   for (size_t k = 0; k < nl; ++k) {
-    HIPCHK(hipEventElapsedTime(&p->pass_ms[k], p->pev[k], p->pev[k + 1]));
     if (p->launches[k].kind != 0) {
-      p->pass_kind[k] = 3;  // not a pass kernel
+      kinds[k] = 3;  // not a pass kernel
       continue;
     }
     const PassDesc& d = p->h_desc[p->launches[k].desc];
-    p->pass_kind[k] = (d.flags & F_REDUCE) ? 1 : (d.flags & F_OBS_DIAG) ? 2 : 0;
+    kinds[k] = (d.flags & F_REDUCE) ? 1 : (d.flags & F_OBS_DIAG) ? 2 : 0;
   }
+  ++p->prof_runs;
   return QG_OK;
 }
 
 extern "C" int qg_plan_set_profiling(qg_plan* p, int on) {
   if (!p) return fail(QG_ERR_INVALID, "NULL plan");
   p->profile = on != 0;
+  p->prof_runs = 0;
+  p->pass_ms.clear();
+  p->pass_kind.clear();
   return QG_OK;
 }
 
 extern "C" int qg_plan_pass_times(const qg_plan* p, int cap, float* ms, int* kinds, int* count) {
   if (!p || !count) return fail(QG_ERR_INVALID, "NULL argument");
+  if (p->prof_runs > 0) {
+    HIPCHK(hipSetDevice(p->ctx->device));
+    p->pass_ms.clear();
+    p->pass_kind.clear();
+    for (size_t r = 0; r < p->prof_runs; ++r) {
+      const std::vector<hipEvent_t>& ev = p->prof_ev[r];
+      const std::vector<int>& kd = p->prof_kind[r];
+      HIPCHK(hipEventSynchronize(ev[kd.size()]));
+      for (size_t k = 0; k < kd.size(); ++k) {
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+        p->pass_ms.push_back(t);
+        p->pass_kind.push_back(kd[k]);
+      }
+    }
+  }
   const int n = (int)p->pass_ms.size();
   *count = n;
   for (int k = 0; k < n && k < cap; ++k) {
@@ -1489,8 +1521,6 @@ static int run_fused(qg_plan* p, int B, size_t bytes_per_circuit, int per_out, b
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipEventElapsedTime(&p->last_ms, c->ev0, c->ev1));
   p->launches.clear();
-  p->pass_ms.clear();
-  p->pass_kind.clear();
   return QG_OK;
 }
 

@@ -166,12 +166,12 @@ def bench_serve(args, sync):
     from qradient_amd import serve
     nburst = max(40, 8 * args.steps)
     burst = serve.poisson_trace(nburst, 1e9, SEED + 1, (12, 28), (4, 64))
-    sat = serve.replay(burst, slots=8)['circuits_per_s']
+    sat = serve.replay(burst, slots=8, progress=100)['circuits_per_s']
     rate = 0.7 * sat
     nreq = max(100, 20 * args.steps)
     trace = serve.poisson_trace(nreq, rate, SEED, (12, 28), (4, 64))
     t0 = time.perf_counter()
-    st = serve.replay(trace, slots=8)
+    st = serve.replay(trace, slots=8, progress=100)
     dt = time.perf_counter() - t0
     return _line('serving grad_run circuits/s with p50/p99 latency (McClean n~U{12..28}, L~U{4..64}, Poisson '
                  'arrivals at 0.7 x the saturation throughput)', st['circuits_per_s'], 'circuits/s', nreq, 0,

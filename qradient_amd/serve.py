@@ -256,18 +256,27 @@ def poisson_trace(num, rate, seed, n_range=(12, 28), L_range=(4, 64)):
     return out
 
 
-def replay(trace, slots=8, budget_bytes=None, dtype='complex64', device=0, time_scale=1.0, **kw):
-    """Submit a trace at its arrival times (scaled) and return the server stats."""
+def replay(trace, slots=8, budget_bytes=None, dtype='complex64', device=0, time_scale=1.0, progress=0, **kw):
+    """Submit a trace at its arrival times (scaled) and return the server stats.
+    ``progress`` > 0 prints a line to stderr every that many submitted requests
+    and every that many results (long traces)."""
+    import sys
     srv = GradServer(device=device, slots=slots, budget_bytes=budget_bytes, dtype=dtype, **kw)
     t0 = time.perf_counter()
     futs = []
-    for (ta, n, L, axes, angles, obs) in trace:
+    for k, (ta, n, L, axes, angles, obs) in enumerate(trace):
         wait = t0 + ta * time_scale - time.perf_counter()
         if wait > 0:
             time.sleep(wait)
         futs.append(srv.submit(Request(n, L, axes, angles, obs)))
-    for f in futs:
+        if progress and (k + 1) % progress == 0:
+            print('replay: %d of %d submitted, %.1f s' % (k + 1, len(trace), time.perf_counter() - t0),
+                  file=sys.stderr, flush=True)
+    for k, f in enumerate(futs):
         f.result()
+        if progress and (k + 1) % progress == 0:
+            print('replay: %d of %d done, %.1f s' % (k + 1, len(futs), time.perf_counter() - t0),
+                  file=sys.stderr, flush=True)
     st = srv.stats()
     srv.close()
     return st

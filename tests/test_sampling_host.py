@@ -50,3 +50,23 @@ def test_edge_probabilities():
     assert sample_projectors([1.0 + 1e-16, -1e-17], [1.0, 1.0], 10, rs) == 0.0
     with pytest.raises(ValueError):
         sample_projectors([0.5], [1.0], 0)
+
+
+@pytest.mark.parametrize('seed,shots', [(0, 1), (1, 13), (2, 200)])
+def test_vectorised_estimates_equal_the_per_term_sampler(seed, shots):
+    """projector_estimates over many states at once, with one uniform draw of
+    all their shots, equals sample_projectors state after state (the same
+    random numbers in the same order, the same sums)."""
+    from qradient_amd.circuit_logic.base import projector_estimates
+    rs = np.random.RandomState(seed)
+    M, K = 6, 5
+    probs = rs.uniform(size=(M, K))
+    probs[0, 0], probs[1, 1] = 0.0, 1.0            # the edges of [0, 1]
+    w = rs.standard_normal(K)
+    one = [sample_projectors(probs[m], w, shots, np.random.RandomState(50 + seed)) for m in range(1)]
+    rs_a = np.random.RandomState(50 + seed)
+    per_state = np.array([sample_projectors(probs[m], w, shots, rs_a) for m in range(M)])
+    rs_b = np.random.RandomState(50 + seed)
+    u = rs_b.uniform(size=(M, K, shots))
+    assert np.array_equal(projector_estimates(probs, w, u), per_state)
+    assert per_state[0] == one[0]

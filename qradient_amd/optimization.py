@@ -1,34 +1,181 @@
 """Optimisers over the GPU grad_run (reference qradient/optimization.py:3-194).
 
-``McCleanOpt`` and ``QaoaOpt`` keep the reference's cost and parameter
-histories and its aliasing: a McClean circuit's ``angles`` is the optimiser's
-parameter array, updated in place each step (optimization.py:59-61, 91).
-The update rules (``Adam``, ``GradientDescent``, ``RateDecayOnPlateau``) are
-the reference's, on [L, n] (McClean) or [p, 2] (QAOA, columns beta, gamma)
-arrays.  Each step costs one grad_run; the update itself is a few hundred
-host FLOPs next to milliseconds of GPU work, so the parameters stay on the
-host.
+Interface kept from the reference: the class names, constructor and ``step``
+signatures, the ``cost_history`` / ``param_history`` arrays, the
+``optimizer_info`` dict, the rule chosen by ``optimizer['name']`` with its
+hyper-parameter keys and defaults, the update equations, the gradient method
+``step`` selects from ``shot_num`` / ``dense_mode`` / ``component_sampling``
+(optimization.py:69-91, 113-129), the aliasing of a McClean circuit's
+``angles`` with the rule's parameter array (optimization.py:59-61, 91), and
+``ValueError`` for an unknown rule or QAOA's sparse sampling mode.
 
-``QaoaBatchOpt`` (new) steps B QAOA circuits at once: one batched grad_run
-(``QaoaBatch``) per step and the same update rule applied to the [B, p, 2]
-parameter block, each circuit with its own state of the rule.
+Structure here: one history recorder (``_History``) shared by every driver;
+the drivers only say where the gradient comes from.  The rules are a
+registry (``RULES``) of small classes over parameter arrays of any shape, so a
+[B, p, 2] block of a batched QAOA run is B independent rule states.  Each
+step costs one grad_run on the GPU; the rule's update is a few hundred host
+FLOPs next to milliseconds of device work, so the parameters stay on the host.
 
-``McCleanOpt.step`` and ``QaoaOpt.step`` dispatch like the reference
-(optimization.py:69-91, 113-129): ``shot_num == 0`` takes grad_run (McClean:
-grad_run_with_component_sampling with ``component_sampling``); ``shot_num >
-0`` takes the shot-based gradients, sample_grad_dense* with ``dense_mode``
-(the default) and, for McClean, sample_grad* without.
+Deliberate difference: ``RateDecayOnPlateau`` starts from an infinite best
+cost (the reference starts from 1e10); the two agree for every cost below
+1e10.
 """
 import numpy as np
 
+# hyper-parameter defaults of the rules (optimization.py:147, 154-156, 172, 180-181)
+DEFAULTS = {'step_size': 1e-3, 'beta1': 0.9, 'beta2': 0.999, 'eps': 1e-8,
+            'plateau_length': 10, 'decay_rate': 0.5}
 
-class ParametrizedCircuitOptimizer:
-    """Base class (optimization.py:3-39).  Not for instantiation."""
+
+def _hyper(hyper_parameters, key):
+    return hyper_parameters.get(key, DEFAULTS[key])
+
+
+# ---------------------------------------------------------------------------
+# update rules (optimization.py:145-194)
+# ---------------------------------------------------------------------------
+class GradientDescentOptimizer:
+    """A rule updating ``parameters`` in place from a gradient (and the cost of
+    the point it was taken at).  ``iter`` counts the updates."""
+
+    def init(self, parameters, hyper_parameters):
+        self.parameters = parameters
+        self.step_size = _hyper(hyper_parameters, 'step_size')
+        self.iter = 0
+
+    def step(self, gradient, new_cost=None):
+        self.iter += 1
+        self.parameters -= self._increment(np.asarray(gradient), new_cost)
+
+    def _increment(self, gradient, new_cost):
+        raise NotImplementedError
+
+
+class Adam(GradientDescentOptimizer):
+    """Adam with bias-corrected moments (optimization.py:151-167); elementwise."""
+
+    def __init__(self, parameters, hyper_parameters):
+        self.init(parameters, hyper_parameters)
+        self.beta1 = _hyper(hyper_parameters, 'beta1')
+        self.beta2 = _hyper(hyper_parameters, 'beta2')
+        self.eps = _hyper(hyper_parameters, 'eps')
+        self.m = np.zeros(np.shape(parameters))
+        self.v = np.zeros(np.shape(parameters))
+        self.m_hat = np.zeros(np.shape(parameters))
+        self.v_hat = np.zeros(np.shape(parameters))
+
+    def _increment(self, g, new_cost):
+        k = self.iter
+        self.m = self.beta1 * self.m + (1 - self.beta1) * g
+        self.v = self.beta2 * self.v + (1 - self.beta2) * g ** 2
+        self.m_hat = self.m / (1 - self.beta1 ** k)
+        self.v_hat = self.v / (1 - self.beta2 ** k)
+        return self.step_size * self.m_hat / (np.sqrt(self.v_hat) + self.eps)
+
+
+class GradientDescent(GradientDescentOptimizer):
+    """Plain descent with the rate ``decay_function(step_size, iter)``
+    (optimization.py:169-175; default: a constant rate)."""
+
+    def __init__(self, parameters, hyper_parameters):
+        self.init(parameters, hyper_parameters)
+        self.decay_function = hyper_parameters.get('decay_function', self._constant_rate)
+
+    @staticmethod
+    def _constant_rate(step_size, iteration):
+        return step_size
+
+    def _increment(self, g, new_cost):
+        return self.decay_function(self.step_size, self.iter) * g
+
+
+class RateDecayOnPlateau(GradientDescentOptimizer):
+    """Descent whose rate is multiplied by ``decay_rate`` once the cost has
+    failed ``plateau_length`` times in a row to reach a new best
+    (optimization.py:177-194).  Counts and the best cost are arrays over the
+    leading axis when ``batched`` (one rule state per circuit)."""
+
+    def __init__(self, parameters, hyper_parameters, batched=False):
+        self.init(parameters, hyper_parameters)
+        self.plateau_length = _hyper(hyper_parameters, 'plateau_length')
+        self.decay_rate = _hyper(hyper_parameters, 'decay_rate')
+        shape = (np.shape(parameters)[0],) if batched else ()
+        self.batched = batched
+        self.step_size = np.full(shape, float(self.step_size)) if batched else self.step_size
+        self.plateau_counter = np.zeros(shape, dtype=int) if batched else 0
+        self.cost = np.full(shape, np.inf) if batched else np.inf
+
+    def _increment(self, g, new_cost):
+        cost = np.asarray(new_cost, dtype=float)
+        stalled = cost > self.cost
+        count = np.where(stalled, self.plateau_counter + 1, 0)
+        decay = stalled & (count >= self.plateau_length)
+        step = np.where(decay, self.step_size * self.decay_rate, self.step_size)
+        count = np.where(decay, 0, count)
+        best = np.where(stalled, self.cost, cost)
+        if self.batched:
+            self.step_size, self.plateau_counter, self.cost = step, count, best
+            return step.reshape((-1,) + (1,) * (g.ndim - 1)) * g
+        self.step_size, self.plateau_counter, self.cost = float(step), int(count), float(best)
+        return self.step_size * g
+
+
+class BatchRateDecayOnPlateau(RateDecayOnPlateau):
+    """RateDecayOnPlateau with a step size, counter and best cost per circuit
+    (the leading axis of the parameter block)."""
+
+    def __init__(self, parameters, hyper_parameters):
+        RateDecayOnPlateau.__init__(self, parameters, hyper_parameters, batched=True)
+
+
+RULES = {'Adam': Adam, 'GradientDescent': GradientDescent, 'RateDecayOnPlateau': RateDecayOnPlateau}
+
+
+def make_rule(optimizer, parameters, batched=False):
+    """The rule named by ``optimizer['name']`` over ``parameters`` (updated in place)."""
+    name = optimizer.get('name') if isinstance(optimizer, dict) else None
+    if name not in RULES:
+        raise ValueError('unknown optimizer {!r}: the name must be one of {}'.format(optimizer, sorted(RULES)))
+    if batched and name == 'RateDecayOnPlateau':
+        return BatchRateDecayOnPlateau(parameters, optimizer)
+    return RULES[name](parameters, optimizer)
+
+
+# ---------------------------------------------------------------------------
+# drivers (optimization.py:3-143)
+# ---------------------------------------------------------------------------
+class _History:
+    """cost_history[k] is the cost at param_history[k]; param_history[0] holds
+    the initial point and step k writes point k + 1."""
+
+    def _open(self, shape, max_iter, ini):
+        self.max_iter = max_iter
+        self.iter = 0
+        self.param_history = np.zeros((max_iter,) + tuple(shape))
+        self.cost_history = np.zeros(max_iter)
+        self.param_history[0] = ini
+
+    def _warn_if_exhausted(self):
+        if self.iter >= self.max_iter:
+            print('{}: all {} iterations of the history are used.'.format(type(self).__name__, self.max_iter))
+
+    def _record(self, cost, gradient):
+        self.cost_history[self.iter] = cost
+        self.optimizer.step(gradient, cost)
+        self.iter += 1
+        self.param_history[self.iter] = self.optimizer.parameters
+
+
+class ParametrizedCircuitOptimizer(_History):
+    """Driver base (optimization.py:3-39): a circuit, its rule and histories."""
 
     def init(self, circuit, max_iter):
         self.circuit = circuit
         self.max_iter = max_iter
         self.iter = 0
+
+    def pick(self, optimizer, ini_parameters):
+        self.optimizer = make_rule(optimizer, ini_parameters)
 
     def step(self):
         pass
@@ -36,108 +183,80 @@ class ParametrizedCircuitOptimizer:
     def reset(self):
         pass
 
-    def pick(self, optimizer, ini_parameters):
-        """The gradient-descent rule named by ``optimizer['name']`` (optimization.py:24-39)."""
-        name = optimizer['name']
-        if name == 'Adam':
-            self.optimizer = Adam(ini_parameters, optimizer)
-        elif name == 'GradientDescent':
-            self.optimizer = GradientDescent(ini_parameters, optimizer)
-        elif name == 'RateDecayOnPlateau':
-            self.optimizer = RateDecayOnPlateau(ini_parameters, optimizer)
-        else:
-            raise ValueError('No optimizer {} known.'.format(optimizer))
-
-    def _check_iter(self):
-        if self.iter >= self.max_iter:
-            print('Maximum amount of iterations reached: {}.'.format(self.max_iter))
-
-
+    def __str__(self):
+        return str(self.optimizer_info)
 
 
 class McCleanOpt(ParametrizedCircuitOptimizer):
-    """optimization.py:41-96.  ``ini_parameters`` (default: the circuit's
-    angles) becomes the circuit's ``angles`` and the optimiser's parameters."""
+    """Optimiser of a McClean circuit's [L, n] angles (optimization.py:41-96).
+    ``ini_parameters`` (default: the circuit's angles) becomes both the
+    circuit's ``angles`` and the rule's parameter array."""
+
+    # (shot-based, dense_mode, component_sampling) -> the circuit's gradient method
+    _METHODS = {(False, True, False): 'grad_run', (False, False, False): 'grad_run',
+                (False, True, True): 'grad_run_with_component_sampling',
+                (False, False, True): 'grad_run_with_component_sampling',
+                (True, True, False): 'sample_grad_dense',
+                (True, True, True): 'sample_grad_dense_with_component_sampling',
+                (True, False, False): 'sample_grad',
+                (True, False, True): 'sample_grad_with_component_sampling'}
 
     def __init__(self, circuit, optimizer, max_iter=1000, **kwargs):
-        ParametrizedCircuitOptimizer.init(self, circuit, max_iter)
-        self.param_history = np.zeros([self.max_iter, circuit.lnum, circuit.qnum], dtype='double')
-        self.cost_history = np.zeros(self.max_iter, dtype='double')
-        ini_parameters = kwargs.get('ini_parameters', circuit.angles)
-        self.param_history[0] = ini_parameters
-        self.circuit.angles = ini_parameters
-        self.optimizer_info = optimizer
-        self.pick(optimizer, ini_parameters)
+        self.init(circuit, max_iter)
+        self._setup(kwargs.get('ini_parameters', circuit.angles), optimizer, max_iter)
 
-    def __str__(self):
-        return self.optimizer_info.__str__()
+    def _setup(self, ini, optimizer, max_iter):
+        self._open((self.circuit.lnum, self.circuit.qnum), max_iter, ini)
+        self.circuit.angles = ini
+        self.optimizer_info = optimizer
+        self.pick(optimizer, ini)
 
     def step(self, shot_num=0, dense_mode=True, component_sampling=False):
-        self._check_iter()
-        if shot_num == 0:
-            if component_sampling:
-                e, g = self.circuit.grad_run_with_component_sampling()
-            else:
-                e, g = self.circuit.grad_run()
-        elif dense_mode:
-            if component_sampling:
-                e, g = self.circuit.sample_grad_dense_with_component_sampling(shot_num=shot_num)
-            else:
-                e, g = self.circuit.sample_grad_dense(shot_num=shot_num)
-        elif component_sampling:
-            e, g = self.circuit.sample_grad_with_component_sampling(shot_num=shot_num)
-        else:
-            e, g = self.circuit.sample_grad(shot_num=shot_num)
-        self.cost_history[self.iter] = e
-        self.optimizer.step(g, e)
-        self.iter += 1
-        self.param_history[self.iter] = self.optimizer.parameters
+        self._warn_if_exhausted()
+        method = getattr(self.circuit, self._METHODS[(shot_num != 0, bool(dense_mode), bool(component_sampling))])
+        e, g = method(shot_num=shot_num) if shot_num != 0 else method()
+        self._record(e, g)
         self.circuit.angles = self.optimizer.parameters
 
     def reset(self, **kwargs):
-        ini_parameters = kwargs.get('ini_parameters', self.param_history[0])
-        optimizer = kwargs.get('optimizer', self.optimizer_info)
-        max_iter = kwargs.get('max_iter', self.max_iter)
-        self.__init__(self.circuit, optimizer, max_iter, ini_parameters=ini_parameters)
+        ini = kwargs.get('ini_parameters', self.param_history[0])
+        self._setup(ini, kwargs.get('optimizer', self.optimizer_info), kwargs.get('max_iter', self.max_iter))
 
 
 class QaoaOpt(ParametrizedCircuitOptimizer):
-    """optimization.py:98-143: parameters [p, 2] with columns (beta, gamma)."""
+    """Optimiser of a QAOA circuit's [p, 2] parameters, columns (beta, gamma)
+    (optimization.py:98-143)."""
 
     def __init__(self, circuit, optimizer, betas, gammas, max_iter=1000):
-        ParametrizedCircuitOptimizer.init(self, circuit, max_iter)
-        self.param_history = np.zeros([self.max_iter, circuit.lnum, 2], dtype='double')
-        self.cost_history = np.zeros(self.max_iter, dtype='double')
-        self.param_history[0] = np.array([betas, gammas]).transpose()
-        self.optimizer_info = optimizer
-        self.pick(optimizer, np.array([betas, gammas]).transpose())
+        self.init(circuit, max_iter)
+        self._setup(betas, gammas, optimizer, max_iter)
 
-    def __str__(self):
-        return self.optimizer_info.__str__()
+    def _setup(self, betas, gammas, optimizer, max_iter):
+        ini = np.stack([np.asarray(betas, dtype=float), np.asarray(gammas, dtype=float)], axis=1)
+        self._open((self.circuit.lnum, 2), max_iter, ini)
+        self.optimizer_info = optimizer
+        self.pick(optimizer, ini)
 
     def step(self, shot_num=0, dense_mode=True):
-        self._check_iter()
-        betas, gammas = self.param_history[self.iter, :, 0], self.param_history[self.iter, :, 1]
+        self._warn_if_exhausted()
+        point = self.param_history[self.iter]
         if shot_num == 0:
-            e, g = self.circuit.grad_run(betas, gammas)
-        elif dense_mode:
-            e, g = self.circuit.sample_grad_dense(betas, gammas, shot_num=shot_num)
+            e, g = self.circuit.grad_run(point[:, 0], point[:, 1])
+        elif not dense_mode:
+            raise ValueError('QaoaOpt.step: only dense_mode=True is implemented for shot-based gradients')
         else:
-            raise ValueError('dense_mode must be True, sparse sampling method is not implmented yet.')
-        self.cost_history[self.iter] = e
-        self.optimizer.step(g, e)
-        self.iter += 1
-        self.param_history[self.iter] = self.optimizer.parameters
+            e, g = self.circuit.sample_grad_dense(point[:, 0], point[:, 1], shot_num=shot_num)
+        self._record(e, g)
 
     def reset(self, **kwargs):
-        if ('betas' in kwargs) and ('gammas' in kwargs):
-            self.param_history[0] = np.array([kwargs['betas'], kwargs['gammas']]).transpose()
-        optimizer = kwargs.get('optimizer', self.optimizer_info)
-        max_iter = kwargs.get('max_iter', self.max_iter)
-        self.__init__(self.circuit, optimizer, self.param_history[0, :, 0], self.param_history[0, :, 1], max_iter)
+        first = self.param_history[0]
+        betas, gammas = kwargs.get('betas'), kwargs.get('gammas')
+        if betas is None or gammas is None:
+            betas, gammas = first[:, 0].copy(), first[:, 1].copy()
+        self._setup(betas, gammas, kwargs.get('optimizer', self.optimizer_info), kwargs.get('max_iter', self.max_iter))
 
 
-class QaoaBatchOpt:
+class QaoaBatchOpt(_History):
     """B QAOA circuits optimised together: one batched grad_run per step.
 
     ``batch`` is a ``QaoaBatch`` plan, ``observables`` its B Hamiltonians
@@ -151,128 +270,16 @@ class QaoaBatchOpt:
         self.batch = batch
         self.observables = batch.pack(observables)
         self.B = len(self.observables)
-        self.max_iter = max_iter
-        self.iter = 0
         p = batch.p
-        betas = np.asarray(betas, dtype='double').reshape(self.B, p)
-        gammas = np.asarray(gammas, dtype='double').reshape(self.B, p)
-        self.param_history = np.zeros([self.max_iter, self.B, p, 2], dtype='double')
-        self.cost_history = np.zeros([self.max_iter, self.B], dtype='double')
-        self.param_history[0, :, :, 0] = betas
-        self.param_history[0, :, :, 1] = gammas
+        ini = np.stack([np.asarray(betas, dtype=float).reshape(self.B, p),
+                        np.asarray(gammas, dtype=float).reshape(self.B, p)], axis=-1)
+        self._open((self.B, p, 2), max_iter, ini)
+        self.cost_history = np.zeros((max_iter, self.B))
         self.optimizer_info = optimizer
-        ini = np.stack([betas, gammas], axis=-1)
-        name = optimizer['name']
-        if name == 'Adam':
-            self.optimizer = Adam(ini, optimizer)
-        elif name == 'GradientDescent':
-            self.optimizer = GradientDescent(ini, optimizer)
-        elif name == 'RateDecayOnPlateau':
-            self.optimizer = BatchRateDecayOnPlateau(ini, optimizer)
-        else:
-            raise ValueError('No optimizer {} known.'.format(optimizer))
+        self.optimizer = make_rule(optimizer, ini, batched=True)
 
     def step(self):
-        if self.iter >= self.max_iter:
-            print('Maximum amount of iterations reached: {}.'.format(self.max_iter))
-        cur = self.param_history[self.iter]
-        e, g = self.batch.grad_run(self.observables, cur[:, :, 0], cur[:, :, 1])
-        self.cost_history[self.iter] = e
-        self.optimizer.step(g, e)
-        self.iter += 1
-        self.param_history[self.iter] = self.optimizer.parameters
-
-
-# ---------------------------------------------------------------------------
-# gradient-descent rules (optimization.py:145-194)
-# ---------------------------------------------------------------------------
-class GradientDescentOptimizer:
-    def init(self, parameters, hyper_parameters):
-        self.parameters = parameters
-        self.step_size = hyper_parameters.get('step_size', .001)
-        self.iter = 0
-
-
-class Adam(GradientDescentOptimizer):
-    """optimization.py:151-167; elementwise, so a [B, p, 2] block is B
-    independent Adam states."""
-
-    def __init__(self, parameters, hyper_parameters):
-        GradientDescentOptimizer.init(self, parameters, hyper_parameters)
-        self.beta1 = hyper_parameters.get('beta1', .9)
-        self.beta2 = hyper_parameters.get('beta2', .999)
-        self.eps = hyper_parameters.get('eps', 10. ** -8)
-        self.m = np.zeros(parameters.shape, dtype='double')
-        self.v = np.zeros(parameters.shape, dtype='double')
-        self.m_hat = np.zeros(parameters.shape, dtype='double')
-        self.v_hat = np.zeros(parameters.shape, dtype='double')
-
-    def step(self, gradient, *args):
-        self.iter += 1
-        self.m = self.beta1 * self.m + (1 - self.beta1) * gradient
-        self.v = self.beta2 * self.v + (1 - self.beta2) * gradient ** 2
-        self.m_hat = self.m / (1 - self.beta1 ** self.iter)
-        self.v_hat = self.v / (1 - self.beta2 ** self.iter)
-        self.parameters -= self.step_size * self.m_hat / (np.sqrt(self.v_hat) + self.eps)
-
-
-class GradientDescent(GradientDescentOptimizer):
-    """optimization.py:169-175: step_size scaled by decay_function(step_size, iter)."""
-
-    def __init__(self, parameters, hyper_parameters):
-        GradientDescentOptimizer.init(self, parameters, hyper_parameters)
-        self.decay_function = hyper_parameters.get('decay_function', lambda step_size, iter: step_size)
-
-    def step(self, gradient, *args):
-        self.iter += 1
-        self.parameters -= self.decay_function(self.step_size, self.iter) * gradient
-
-
-class RateDecayOnPlateau(GradientDescentOptimizer):
-    """optimization.py:177-194: the step size is multiplied by decay_rate after
-    plateau_length steps without a new lowest cost."""
-
-    def __init__(self, parameters, hyper_parameters):
-        GradientDescentOptimizer.init(self, parameters, hyper_parameters)
-        self.plateau_length = hyper_parameters.get('plateau_length', 10)
-        self.decay_rate = hyper_parameters.get('decay_rate', .5)
-        self.plateau_counter = 0
-        self.cost = 10. ** 10
-
-    def step(self, gradient, new_cost):
-        self.iter += 1
-        if new_cost > self.cost:
-            self.plateau_counter += 1
-            if self.plateau_counter >= self.plateau_length:
-                self.step_size *= self.decay_rate
-                self.plateau_counter = 0
-        else:
-            self.cost = new_cost
-            self.plateau_counter = 0
-        self.parameters -= self.step_size * gradient
-
-
-class BatchRateDecayOnPlateau(GradientDescentOptimizer):
-    """RateDecayOnPlateau with one step size, counter and best cost per circuit
-    (leading axis of the parameter block)."""
-
-    def __init__(self, parameters, hyper_parameters):
-        GradientDescentOptimizer.init(self, parameters, hyper_parameters)
-        B = parameters.shape[0]
-        self.plateau_length = hyper_parameters.get('plateau_length', 10)
-        self.decay_rate = hyper_parameters.get('decay_rate', .5)
-        self.step_size = np.full(B, float(self.step_size))
-        self.plateau_counter = np.zeros(B, dtype=int)
-        self.cost = np.full(B, 10. ** 10)
-
-    def step(self, gradient, new_cost):
-        self.iter += 1
-        new_cost = np.asarray(new_cost, dtype='double')
-        worse = new_cost > self.cost
-        self.plateau_counter = np.where(worse, self.plateau_counter + 1, 0)
-        decay = worse & (self.plateau_counter >= self.plateau_length)
-        self.step_size = np.where(decay, self.step_size * self.decay_rate, self.step_size)
-        self.plateau_counter = np.where(decay, 0, self.plateau_counter)
-        self.cost = np.where(worse, self.cost, new_cost)
-        shape = (-1,) + (1,) * (self.parameters.ndim - 1)
-        self.parameters -= self.step_size.reshape(shape) * gradient
+        self._warn_if_exhausted()
+        point = self.param_history[self.iter]
+        e, g = self.batch.grad_run(self.observables, point[..., 0], point[..., 1])
+        self._record(e, g)

@@ -16,6 +16,18 @@ Files (numpy .npz, no pickled objects):
   qaoa_n8_p3_reg3.npz        QAOA 8 qubits, p = 3, a seeded 3-regular MaxCut graph
   qaoa_reg3_n20_b512.npz     the batched QAOA workload: 512 seeded 3-regular graphs on
                              20 vertices (edges a < b), betas and gammas in [0, pi)
+  mcclean_n24_L40_zzchain.npz  the headline McClean workload: 24 qubits, 40 layers (960
+                             parameters), seed 2024 (axes, angles, then the 23 N(0,1) ZZ-chain
+                             weights); E and the gradient from the C restatement of the
+                             reference algorithm (stored history), cross-checked against the
+                             matrix-free numpy adjoint (both recorded, with their difference)
+  qaoa_reg3_n20_b512_expected.npz  E[512] and grad[512, 8, 2] of the batched QAOA workload
+                             (complex128), from the C restatement; the first circuits are
+                             cross-checked against the numpy oracle
+  serve_trace_seed3.npz      the 14-request serving trace poisson_trace(14, 50.0, seed=3,
+                             n in {12..28}, L in {4..64}) (requests of 13 to 28 qubits, 9 to
+                             64 layers): for its requests of at most 20 qubits, their
+                             inputs, E and gradient (C restatement, complex128)
   sampled_n5.npz             the sampled gradients (mc_clean.py:80-350, qaoa.py:83-158) of a
                              McClean 5-qubit, 2-layer circuit (positive ZZ-chain weights) and a
                              QAOA 5-qubit, p = 2 MaxCut circuit, each from numpy's global
@@ -33,6 +45,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, '..', '..'))
 
 from oracle import statevector as sv  # noqa: E402
+from oracle import c_oracle  # noqa: E402
 
 
 def z0z1(n):
@@ -129,10 +142,107 @@ def sampled_n5():
     np.savez(os.path.join(HERE, 'sampled_n5.npz'), **res)
 
 
+def mcclean_cfg2_workload():
+    """McClean 24 qubits, 40 layers, N(0,1) ZZ chain, seed 2024."""
+    n, L = 24, 40
+    rs = np.random.RandomState(2024)
+    axes, angles = sv.mcclean_random_params(L, n, rs)
+    w = rs.standard_normal(n - 1)
+    return n, L, axes, angles, w
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def mcclean_cfg2():
+    import concurrent.futures
+    n, L, axes, angles, w = mcclean_cfg2_workload()
+    obs = sv.zz_chain_observable(n, w)
+    # the two formulations run side by side (C with its 41-state history,
+    # numpy matrix-free); their agreement is recorded with the values
+    with concurrent.futures.ProcessPoolExecutor(2) as ex:
+        fc = ex.submit(c_oracle.mcclean_grad_run, n, axes, angles, obs)
+        fa = ex.submit(sv.mcclean_grad_run_adjoint, n, axes, angles, obs)
+        E, g = fc.result()
+        Ea, ga = fa.result()
+    err = _rel(ga, g)
+    assert err <= 1e-12 and abs(Ea - E) <= 1e-12 * max(abs(E), 1.0), (err, E, Ea)
+    np.savez(os.path.join(HERE, 'mcclean_n24_L40_zzchain.npz'), axes=axes.astype(np.int32), angles=angles,
+             weights=w, E=np.float64(E), grad=g, E_adjoint=np.float64(Ea), grad_adjoint=ga,
+             formulation_rel_err=np.float64(err))
+    print('mcclean_n24_L40: E', E, 'C vs numpy adjoint rel err', err)
+
+
+def _qaoa_one(args):
+    n, edges, betas, gammas = args
+    return c_oracle.qaoa_grad_run(n, sv.maxcut_observable(n, edges), betas, gammas)
+
+
+def qaoa_batch_expected(workers=6):
+    import concurrent.futures
+    f = np.load(os.path.join(HERE, 'qaoa_reg3_n20_b512.npz'))
+    edges, betas, gammas = f['edges'], f['betas'], f['gammas']
+    B, p = betas.shape
+    n = 20
+    with concurrent.futures.ProcessPoolExecutor(workers) as ex:
+        res = list(ex.map(_qaoa_one, [(n, edges[b], betas[b], gammas[b]) for b in range(B)], chunksize=8))
+    E = np.array([r[0] for r in res])
+    grad = np.array([r[1] for r in res])
+    worst = 0.0
+    for b in range(4):
+        Ea, ga = sv.qaoa_grad_run_adjoint(n, sv.maxcut_observable(n, edges[b]), betas[b], gammas[b])
+        worst = max(worst, _rel(ga, grad[b]), abs(Ea - E[b]) / max(abs(E[b]), 1.0))
+    assert worst <= 1e-12, worst
+    np.savez(os.path.join(HERE, 'qaoa_reg3_n20_b512_expected.npz'), E=E, grad=grad,
+             crosscheck_rel_err=np.float64(worst))
+    print('qaoa512: cross-check of 4 circuits against the numpy oracle', worst)
+
+
+SERVE_TRACE = dict(num=14, rate=50.0, seed=3, n_range=(12, 28), L_range=(4, 64))
+
+
+def _serve_one(args):
+    n, axes, angles, w = args
+    return c_oracle.mcclean_grad_run(n, axes, angles, sv.zz_chain_observable(n, w))
+
+
+def serve_trace_expected(workers=6):
+    import concurrent.futures
+    from qradient_amd.serve import poisson_trace
+    t = SERVE_TRACE
+    trace = poisson_trace(t['num'], t['rate'], t['seed'], t['n_range'], t['L_range'])
+    idx = [k for k, r in enumerate(trace) if r[1] <= 20]
+    jobs = []
+    for k in idx:
+        _, n, L, axes, angles, obs = trace[k]
+        w = np.array([obs['zz'][i, i + 1] for i in range(n - 1)], dtype=float)
+        jobs.append((n, axes, angles, w))
+    with concurrent.futures.ProcessPoolExecutor(workers) as ex:
+        res = list(ex.map(_serve_one, jobs))
+    out = {'index': np.array(idx, dtype=np.int32)}
+    for k, (n, axes, angles, w), (E, g) in zip(idx, jobs, res):
+        out['axes_%d' % k] = axes.astype(np.int32)
+        out['angles_%d' % k] = angles
+        out['E_%d' % k] = np.float64(E)
+        out['grad_%d' % k] = g
+    # one request cross-checked against the numpy matrix-free adjoint
+    n, axes, angles, w = jobs[0]
+    Ea, ga = sv.mcclean_grad_run_adjoint(n, axes, angles, sv.zz_chain_observable(n, w))
+    out['crosscheck_rel_err'] = np.float64(_rel(ga, res[0][1]))
+    assert out['crosscheck_rel_err'] <= 1e-12
+    np.savez(os.path.join(HERE, 'serve_trace_seed3.npz'), **out)
+    print('serve trace: requests', idx, 'cross-check', float(out['crosscheck_rel_err']))
+
+
+ALL = {'mcclean_cfg1': mcclean_cfg1, 'mcclean_n14': mcclean_n14, 'qaoa_small': qaoa_small,
+       'qaoa_batch_workload': qaoa_batch_workload, 'sampled_n5': sampled_n5,
+       'mcclean_cfg2': mcclean_cfg2, 'qaoa_batch_expected': qaoa_batch_expected,
+       'serve_trace_expected': serve_trace_expected}
+
+
 if __name__ == '__main__':
-    mcclean_cfg1()
-    mcclean_n14()
-    qaoa_small()
-    qaoa_batch_workload()
-    sampled_n5()
+    # no argument: every fixture; otherwise the named ones
+    for name in (sys.argv[1:] or list(ALL)):
+        ALL[name]()
     print('fixtures written to', HERE)

@@ -7,15 +7,27 @@ gradients), inputs resident on the device.  ``value`` is the complex64-state
 (FP64 arithmetic and reductions) throughput in circuits/s; the plain
 complex128 method is timed on the same circuit and reported beside it.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ...]
 
-Prints one JSON line.  ``roofline`` is measured live: HIP events around every
-pass launch of the timed steps (on the library's stream) give the average
-duration of the dominant kernel; its algorithmic bytes per launch come from
-the pass's state reads and writes.  ``cpu_baseline`` times the C restatement
-of the reference algorithm (oracle/c/oracle_ref.c) on one core over a bounded
-sample (the first layer of the same circuit) and scales it to the full
-circuit.
+Prints one JSON line.  ``roofline`` follows the algorithmic byte count of the
+adjoint method, each statevector counted once per layer: per layer the
+forward sweep reads and writes psi (2 S) and the reverse sweep reads and
+writes psi and lambda (4 S), plus S for the initial state and 2 S for
+lambda = O psi with E, so B_alg = (6 L + 3) S for the whole run (S = 2^n
+amplitudes x bytes per amplitude).  A launch that covers one of the P passes
+of a layer is credited 1/P of its layer's bytes.  ``roofline.achieved`` is
+the dominant kernel's algorithmic bytes per launch over its average launch
+time (HIP events on the library's stream around every launch of the timed
+steps); ``roofline.whole_step`` is B_alg over ms_per_step.  ``hbm`` holds the
+bytes the kernel actually moves per launch (rocprofv3 FETCH_SIZE +
+WRITE_SIZE counters of the same workload, from the committed profile named in
+``hbm.traffic_source``: not measured in this run) and the rate they give.
+``copy_ceiling`` is measured in the run: a streaming copy between two 1 GiB
+buffers (HBM) and between two buffers of one state's size (which the 256 MB
+memory-side cache can hold).  ``cpu_baseline`` times the C restatement of the
+reference algorithm (oracle/c/oracle_ref.c) on one core over a bounded sample
+(the first 3 layers of the same circuit, median of 3 runs) and scales it to
+the full circuit.
 """
 import argparse
 import ctypes
@@ -74,33 +86,77 @@ def time_steps(circ, steps, warmup, sync, profile):
     return dt, E, g, per
 
 
-def roofline(per, state_bytes, steps):
-    """Dominant pass class: reverse two-state passes (read A, read B, write B).
-    ``per``: (ms, kinds) of every launch of the ``steps`` timed steps."""
+def roofline(per, state_bytes, steps, layers):
+    """Dominant launch class of the timed steps and its algorithmic rate.
+
+    ``per``: (ms, kinds) of every launch of the ``steps`` timed steps; kinds
+    0 forward pass, 1 adjoint two-state pass, 2 last forward pass fused with
+    lambda = O psi and E.  With P passes per layer, a forward launch is
+    credited 2 S / P, an adjoint launch 4 S / P and the fused one 2 S / P + 2 S.
+    """
     ms = np.concatenate([p[0] for p in per])
     kinds = np.concatenate([p[1] for p in per])
-    alg = {0: 2 * state_bytes, 1: 3 * state_bytes, 2: 3 * state_bytes}
+    P = int(round(float(np.sum(kinds == 1)) / (steps * layers)))
+    S = state_bytes
+    alg = {0: 2 * S / P, 1: 4 * S / P, 2: 2 * S / P + 2 * S}
     totals = {k: ms[kinds == k].sum() for k in set(kinds.tolist())}
-    dom = max(totals, key=totals.get)
+    dom = max((k for k in totals if k in alg), key=totals.get)
     avg_ms = float(ms[kinds == dom].mean())
     achieved = alg[dom] / (avg_ms * 1e-3) / 1e9
-    return dom, avg_ms, achieved, alg[dom], {int(k): float(v) / steps for k, v in totals.items()}
+    return dict(kind=dom, avg_ms=avg_ms, achieved=achieved, alg_bytes=alg[dom], passes_per_layer=P,
+                ms_by_kind={int(k): float(v) / steps for k, v in totals.items()})
 
 
-def cpu_baseline(axes, angles, obs):
+def copy_ceiling(state_bytes, sync):
+    """Streaming-copy rates on this device (qg_stream_copy, ordinary loads and
+    stores, grid 8 x CUs): between two 1 GiB buffers (HBM-bound), and between
+    two buffers of one state's size ping-ponged (the memory-side cache holds
+    such a pair at 24 qubits in complex64)."""
+    import torch
+    from qradient_amd import _native
+    ctx = _native.Context.default(0)
+    out = {}
+    for name, nbytes in (('hbm_1gib', 1 << 30), ('state_pingpong', int(state_bytes))):
+        a = torch.empty(nbytes // 4, dtype=torch.float32, device='cuda')
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        sync()
+        ms = ctypes.c_double()
+        best = None
+        for r in range(6):
+            src, dst = (a, b) if r % 2 == 0 else (b, a)
+            _native.call('qg_stream_copy', ctx.handle, ctypes.c_void_p(dst.data_ptr()),
+                         ctypes.c_void_p(src.data_ptr()), nbytes, 0, 8, 4, ctypes.byref(ms))
+            if r >= 2:
+                best = ms.value if best is None else min(best, ms.value)
+        out[name + '_gbs'] = 2 * nbytes / (best * 1e-3) / 1e9
+        del a, b
+    torch.cuda.empty_cache()
+    return out
+
+
+def cpu_baseline(axes, angles, obs, layers=3, runs=3):
+    """The C restatement of the reference algorithm on one core: the first
+    ``layers`` layers of the circuit (forward with stored history, then the
+    adjoint sweep of those layers), median of ``runs`` runs, scaled to the
+    circuit's layers."""
+    import os as _os
     from threadpoolctl import threadpool_limits
     from oracle import c_oracle
-    layers = 1
+    times = []
     with threadpool_limits(1):
-        t0 = time.perf_counter()
-        c_oracle.mcclean_grad_run(N_QUBITS, axes, angles, obs, layers=layers)
-        dt = time.perf_counter() - t0
-    per_circuit = dt * N_LAYERS / layers
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            c_oracle.mcclean_grad_run(N_QUBITS, axes, angles, obs, layers=layers)
+            times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    per_circuit = med * N_LAYERS / layers
     return {'value': 1.0 / per_circuit, 'unit': 'circuits/s', 'cores': 1, 'kind': 'port',
-            'sample': 'first {} of {} layers of the same 24-qubit circuit (forward + adjoint sweep of that '
-                      'layer, reference algorithm with stored history), C restatement oracle/c/oracle_ref.c, '
-                      '1 thread; {:.1f} s measured, scaled x{} to the full circuit'.format(layers, N_LAYERS, dt,
-                                                                                        N_LAYERS // layers)}
+            'sample': 'first {} of {} layers of the same 24-qubit circuit (forward with stored history + adjoint '
+                      'sweep of those layers, the reference algorithm), C restatement oracle/c/oracle_ref.c on 1 '
+                      'thread; median of {} runs {:.2f} s (runs: {}), scaled x{:.2f} to the full circuit; host has '
+                      '{} cores'.format(layers, N_LAYERS, runs, med, ', '.join('%.2f' % t for t in times),
+                                        N_LAYERS / layers, len(_os.sched_getaffinity(0)))}
 
 
 def _line(metric, value, unit, steps, warmup, ms, dtype, config, **extra):
@@ -123,8 +179,12 @@ def bench_qaoa(args, sync):
     gammas = wl['gammas']
     betas = wl['betas']
     res = {}
+    shapes = {}
     for dtype in ('complex64', 'complex128'):
-        qb = QaoaBatch(n, p, B, dtype=dtype)
+        # the autotuner's batch shape (tile, chunk, workgroups per CU): cached,
+        # else the committed MI355X default, else timed now
+        qb = QaoaBatch(n, p, B, dtype=dtype, tune=True)
+        shapes[dtype] = qb.shape
         packed = qb.pack(obs)
         steps = args.steps if dtype == 'complex64' else max(2, args.steps // 2)
         for _ in range(args.warmup):
@@ -148,12 +208,28 @@ def bench_qaoa(args, sync):
                  'reductions)', B * 1e3 / c64['ms'], 'circuits/s', c64['steps'], args.warmup, c64['ms'],
                  'c64 (FP64 arithmetic and reductions)',
                  {'workload': 'qaoa_maxcut_reg3_n20_p8_batch512', 'qubits': n, 'layers': p, 'batch': B,
-                  'seed': SEED, 'graphs': 'tests/golden/qaoa_reg3_n20_b512.npz'},
+                  'seed': SEED, 'graphs': 'tests/golden/qaoa_reg3_n20_b512.npz', 'tuned_shape': shapes['complex64']},
                  device_ms_per_step=c64['dev_ms'],
-                 c128_plain={'circuits_per_s': B * 1e3 / c128['ms'], 'ms_per_step': c128['ms']},
+                 c128_plain={'circuits_per_s': B * 1e3 / c128['ms'], 'ms_per_step': c128['ms'],
+                             'tuned_shape': shapes['complex128']},
                  c64_vs_c128_grad_rel_err_max=float(rel.max()),
+                 grad_rel_err_vs_oracle=qaoa_oracle_errors(res),
                  effective_gbs=b_alg / (c64['dev_ms'] * 1e-3) / 1e9,
                  b_alg_bytes=b_alg)
+
+
+def qaoa_oracle_errors(res):
+    """Worst norm-relative gradient error over the 512 circuits of each dtype
+    against the oracle fixture (tests/golden/qaoa_reg3_n20_b512_expected.npz)."""
+    path = os.path.join(ROOT, 'tests', 'golden', 'qaoa_reg3_n20_b512_expected.npz')
+    if not os.path.exists(path):
+        return None
+    ref = np.load(path, allow_pickle=False)['grad']
+    out = {}
+    for dt, r in res.items():
+        d = (r['g'] - ref).reshape(ref.shape[0], -1)
+        out[dt] = float(np.max(np.linalg.norm(d, axis=1) / np.linalg.norm(ref.reshape(ref.shape[0], -1), axis=1)))
+    return out
 
 
 def bench_serve(args, sync):
@@ -196,9 +272,14 @@ def bench_large(args, sync):
     from qradient_amd import _native
     res = {}
     for dtype in ('complex64', 'complex128'):
-        # the library picks the checkpoint period: the smallest that fits HBM;
-        # one untimed run first (plan, workspace allocation, first touch)
-        circ = McClean(n, obs, L, axes=axes, angles=angles, dtype=dtype)
+        # the autotuner's checkpoint period for this shape (cached, else the
+        # committed MI355X default, else timed now); one untimed run first
+        # (plan, workspace allocation, first touch)
+        from qradient_amd import autotune
+        k_tuned = autotune.lookup(autotune.ckpt_key(n, L, dtype, 'cnot'))
+        if k_tuned is None:
+            k_tuned, _ = autotune.tune_ckpt(n, L, dtype)
+        circ = McClean(n, obs, L, axes=axes, angles=angles, dtype=dtype, ckpt_every=int(k_tuned))
         circ.grad_run()
         sync()
         t0 = time.perf_counter()
@@ -219,7 +300,7 @@ def bench_large(args, sync):
                  'checkpoints every %d layers)' % c64['k'], 1e3 / c64['ms'], 'circuits/s', 1, 0, c64['ms'],
                  'c64 (FP64 arithmetic and reductions)',
                  {'workload': 'mcclean_grad_run_n30_L100_zzchain', 'qubits': n, 'layers': L, 'seed': SEED,
-                  'ckpt_every': c64['k'], 'fp64_lds_tiles': bool(c64['lds64'])},
+                  'ckpt_every': c64['k'], 'ckpt_every_tuned': True, 'fp64_lds_tiles': bool(c64['lds64'])},
                  c128_plain={'ms_per_step': c128['ms'], 'ckpt_every': c128['k']},
                  c64_vs_c128_grad_rel_err=rel, effective_gbs=b_alg / (c64['ms'] * 1e-3) / 1e9, b_alg_bytes=b_alg)
 
@@ -294,11 +375,12 @@ def main():
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--workload', default='mcclean24',
-                    choices=['mcclean24', 'qaoa512', 'serve', 'mcclean30', 'sample24'],
+                    choices=['mcclean24', 'qaoa512', 'serve', 'mcclean30', 'sample24', 'stream'],
                     help='mcclean24 (default, the headline line); qaoa512: 512 batched QAOA MaxCut grad_runs '
                          '(20 qubits, p=8); serve: Poisson request stream through the serving loop; mcclean30: '
                          'McClean 30 qubits, 100 layers, complex64 checkpoints vs complex128; sample24: '
-                         'shot-based parameter-shift gradient, 24 qubits, 8 layers')
+                         'shot-based parameter-shift gradient, 24 qubits, 8 layers; stream: the streaming-copy '
+                         'ceiling')
     ap.add_argument('--probe', action='store_true',
                     help='short run for hardware-counter collection: two complex64 grad_runs of the workload '
                          'and one plain 1 GiB device copy (the calibration kernel for FETCH_SIZE / WRITE_SIZE)')
@@ -306,6 +388,11 @@ def main():
     if args.gpus != 1:
         print('bench.py measures one GPU: this build has no multi-GPU path (every configuration is a '
               'single-GPU workload); run with --gpus 1', file=sys.stderr)
+        sys.exit(2)
+
+    if args.workload == 'mcclean24' and args.steps > 256:
+        print('bench.py: --steps above 256 exceeds the per-launch profile a plan keeps (its latest 256 runs), '
+              'from which the roofline is read', file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -316,7 +403,7 @@ def main():
 
     if args.workload != 'mcclean24':
         print(json.dumps({'qaoa512': bench_qaoa, 'serve': bench_serve, 'mcclean30': bench_large,
-                          'sample24': bench_sample}[args.workload](args, sync)))
+                          'sample24': bench_sample, 'stream': bench_stream}[args.workload](args, sync)))
         return
     axes, angles, obs = workload()
     if args.probe:
@@ -339,17 +426,26 @@ def main():
         steps = args.steps if dtype == 'complex64' else max(2, args.steps // 2)
         dt, E, g, per = time_steps(circ, steps, args.warmup, sync, profile=True)
         sb = (2 ** N_QUBITS) * (8 if dtype == 'complex64' else 16)
-        res[dtype] = dict(ms=dt / steps * 1e3, E=E, roof=roofline(per, sb, steps), steps=steps)
+        res[dtype] = dict(ms=dt / steps * 1e3, E=E, roof=roofline(per, sb, steps, N_LAYERS), steps=steps,
+                          mode=plan_mode(circ), sbytes=sb)
         grads[dtype] = g
         circ._destroy_plan()
     c64, c128 = res['complex64'], res['complex128']
     rel = float(np.linalg.norm(grads['complex64'] - grads['complex128']) / np.linalg.norm(grads['complex128']))
-    dom, avg_ms, achieved, alg_bytes, totals = c64['roof']
-    traffic = None
+    rf = c64['roof']
+    S = c64['sbytes']
+    b_alg = (6 * N_LAYERS + 3) * S
+    traffic, tsrc = None, None
     tfile = os.path.join(ROOT, 'profiles', 'traffic_mcclean_n24_L40.json')
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get('c64_reverse_pass_bytes')
+            tj = json.load(f)
+        traffic = tj.get({0: 'c64_forward_pass_bytes', 1: 'c64_reverse_pass_bytes'}.get(rf['kind'], ''))
+        tsrc = ('profiles/traffic_mcclean_n24_L40.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of this workload, '
+                'scaled by a copy-kernel calibration; committed with the kernels it measured, not measured in '
+                'this run')
+    oracle_err = fixture_errors(grads)
+    ceiling = copy_ceiling(S, sync)
     out = {
         'metric': 'grad_run circuits/s (McClean 24 qubits, 40 layers, complex64 state + FP64 reductions)',
         'value': 1e3 / c64['ms'],
@@ -364,23 +460,84 @@ def main():
         'dtype': 'c64 (FP64 arithmetic and reductions)',
         'data': 'synthetic (seeded random McClean circuit and N(0,1) ZZ-chain weights)',
         'config': {'workload': 'mcclean_grad_run_n24_L40_zzchain', 'qubits': N_QUBITS, 'layers': N_LAYERS,
-                   'parameters': N_QUBITS * N_LAYERS, 'entangler': 'cnot', 'seed': SEED},
+                   'parameters': N_QUBITS * N_LAYERS, 'entangler': 'cnot', 'seed': SEED,
+                   'memory_mode': c64['mode']},
         'c128_plain': {'circuits_per_s': 1e3 / c128['ms'], 'ms_per_step': c128['ms'], 'steps': c128['steps'],
-                       'roofline_achieved_gbs': c128['roof'][2]},
+                       'memory_mode': c128['mode'],
+                       'roofline_achieved_gbs': c128['roof']['achieved'],
+                       'whole_step_gbs': (6 * N_LAYERS + 3) * c128['sbytes'] / (c128['ms'] * 1e-3) / 1e9},
         'speedup_c64_vs_c128': c128['ms'] / c64['ms'],
         'c64_vs_c128_grad_rel_err': rel,
+        'grad_rel_err_vs_oracle': oracle_err,
         'E': c64['E'],
-        'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic,
-                     'kernel': {0: 'forward pass', 1: 'adjoint two-state pass', 2: 'forward + observable pass'}[dom],
-                     'avg_launch_ms': avg_ms, 'algorithmic_bytes_per_launch': alg_bytes,
-                     # measured HBM bytes of the kernel over its live launch time
-                     'traffic_gbs': (traffic / (avg_ms * 1e-3) / 1e9) if traffic else None,
-                     'ms_per_step_by_kind': totals},
+        'roofline': {'bound': 'hbm', 'achieved': rf['achieved'], 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': rf['achieved'] / HBM_PEAK_GBS, 'traffic': traffic,
+                     'kernel': {0: 'forward pass', 1: 'adjoint two-state pass', 2: 'forward + observable pass'}[
+                         rf['kind']],
+                     'avg_launch_ms': rf['avg_ms'], 'algorithmic_bytes_per_launch': rf['alg_bytes'],
+                     'passes_per_layer': rf['passes_per_layer'],
+                     'ms_per_step_by_kind': rf['ms_by_kind'],
+                     'whole_step': {'b_alg_bytes': b_alg, 'gbs': b_alg / (c64['ms'] * 1e-3) / 1e9,
+                                    'frac': b_alg / (c64['ms'] * 1e-3) / 1e9 / HBM_PEAK_GBS}},
+        'hbm': {'traffic_bytes_per_launch': traffic,
+                'sustained_gbs': (traffic / (rf['avg_ms'] * 1e-3) / 1e9) if traffic else None,
+                'sustained_frac': (traffic / (rf['avg_ms'] * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
+                'traffic_source': tsrc},
+        'copy_ceiling': ceiling,
     }
     if not args.no_cpu_baseline:
         out['cpu_baseline'] = cpu_baseline(axes, angles, obs)
     print(json.dumps(out))
+
+
+def plan_mode(circ):
+    """'history' or 'checkpoints every k layers' of a McClean plan."""
+    from qradient_amd import _native
+    k = ctypes.c_int32()
+    _native.call('qg_plan_ckpt_every', circ._get_plan(), ctypes.byref(k))
+    return 'history' if k.value == 0 else 'checkpoints every %d layers' % k.value
+
+
+def fixture_errors(grads):
+    """Norm-relative gradient error of each dtype against the oracle fixture of
+    this workload (tests/golden/mcclean_n24_L40_zzchain.npz), when present."""
+    path = os.path.join(ROOT, 'tests', 'golden', 'mcclean_n24_L40_zzchain.npz')
+    if not os.path.exists(path):
+        return None
+    ref = np.load(path, allow_pickle=False)['grad']
+    return {dt: float(np.linalg.norm(g - ref) / np.linalg.norm(ref)) for dt, g in grads.items()}
+
+
+def bench_stream(args, sync):
+    """Streaming-copy bandwidth (qg_stream_copy): the copy ceiling of this device.
+    Sizes from 16 MiB to 1 GiB per buffer, ordinary and non-temporal accesses,
+    buffers ping-ponged (the memory-side cache can serve pairs up to its 256 MB)."""
+    import torch
+    from qradient_amd import _native
+    ctx = _native.Context.default(0)
+    rows = []
+    big = 1 << 30
+    a = torch.empty(big // 4, dtype=torch.float32, device='cuda')
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    b.fill_(2.0)
+    sync()
+    for nt in (0, 1):
+        for mib in (16, 64, 128, 256, 1024):
+            nbytes = mib << 20
+            ms = ctypes.c_double()
+            best = None
+            for r in range(max(4, args.steps)):
+                src, dst = (a, b) if r % 2 == 0 else (b, a)
+                _native.call('qg_stream_copy', ctx.handle, ctypes.c_void_p(dst.data_ptr()),
+                             ctypes.c_void_p(src.data_ptr()), nbytes, nt, 8, 4, ctypes.byref(ms))
+                best = ms.value if best is None else min(best, ms.value)
+            rows.append({'mib': mib, 'nontemporal': nt, 'ms': best, 'gbs': 2 * nbytes / (best * 1e-3) / 1e9})
+    hbm = max(r['gbs'] for r in rows if r['mib'] == 1024)
+    return _line('streaming copy GB/s (read + write, 1 GiB buffers)', hbm, 'GB/s', max(4, args.steps), 0,
+                 None, 'u8', {'workload': 'stream_copy', 'grid_per_cu': 8, 'bytes_per_lane': 16},
+                 sweep=rows, roofline={'bound': 'hbm', 'achieved': hbm, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                                       'frac': hbm / HBM_PEAK_GBS, 'traffic': None, 'kernel': 'k_stream_copy'})
 
 
 if __name__ == '__main__':

@@ -34,6 +34,7 @@ extern "C" {
 #define QG_ERR_UNSUPPORTED 2 /* valid request this build does not support */
 #define QG_ERR_HIP 3         /* HIP runtime failure */
 #define QG_ERR_NOMEM 4       /* device memory exhausted */
+#define QG_ERR_PENDING 5     /* qg_poll / qg_wait_any: not finished within the timeout */
 
 #define QG_C128 0 /* complex128 state, FP64 arithmetic ("plain" method) */
 #define QG_C64 1  /* complex64 state, FP64 arithmetic and FP64 reductions */
@@ -76,13 +77,35 @@ int qg_ctx_create(int device, void* stream, qg_ctx** out);
 int qg_ctx_destroy(qg_ctx* ctx);
 int qg_ctx_synchronize(qg_ctx* ctx);
 
+/* Measurement utility, no counterpart in the reference: `reps` copies of
+ * `bytes` (a multiple of 16) from the device buffer src to dst on the
+ * context's stream by a streaming-copy kernel (16 bytes per lane, four
+ * accesses in flight per lane, grid = grid_per_cu x the CU count; nontemporal
+ * = 1 uses streaming loads and stores).  *ms_out = device milliseconds per
+ * copy (HIP events).  The benchmark's copy ceiling for the pass kernels. */
+int qg_stream_copy(qg_ctx* ctx, void* dst, const void* src, size_t bytes, int nontemporal, int grid_per_cu,
+                   int reps, double* ms_out);
+
 /* ---- McClean circuit (qradient/circuit_logic/mc_clean.py) ----------------
  * Plan for n qubits and L layers.  ckpt_every = 0 lets the planner keep the
- * per-pass state history when it fits in HBM and otherwise checkpoint; k > 0
- * forces checkpoints every k layers with psi uncomputed in between;
- * QG_HISTORY forces the history (QG_ERR_NOMEM if it does not fit).
- * Replaces the construction of McClean.__init__ (mc_clean.py:9-25). */
+ * per-pass state history when it fits in 75% of the device's total memory
+ * and otherwise checkpoint (the smallest period whose states fit that
+ * budget); k > 0 forces checkpoints every k layers with psi uncomputed in
+ * between; QG_HISTORY forces the history (QG_ERR_NOMEM if it does not fit).
+ * The choice depends on the shape and the device only, never on the memory
+ * free at the time, so one shape always gives the same result bits;
+ * qg_plan_ckpt_every reports it.  Replaces the construction of
+ * McClean.__init__ (mc_clean.py:9-25). */
 int qg_mcclean_plan(qg_ctx* ctx, int n, int L, int dtype, int entangler, int ckpt_every, qg_plan** out);
+
+/* Device workspace (bytes) a McClean plan of this shape uses for a grad run
+ * without ini_state (0 for n <= 13: one workgroup runs the whole circuit),
+ * and the memory mode it chooses (*ckpt_chosen = 0: the per-pass history,
+ * k > 0: checkpoints every k layers; may be NULL).  The same layout rule as
+ * qg_mcclean_plan.  ctx may be NULL when ckpt_every is QG_HISTORY or k > 0
+ * (the rule then needs no device).  Used to admit requests by footprint. */
+int qg_mcclean_workspace_bytes(qg_ctx* ctx, int n, int L, int dtype, int ckpt_every, size_t* bytes,
+                               int* ckpt_chosen);
 
 /* McClean.grad_run (mc_clean.py:47-78).  axes: int32 [L*n] in {0,1,2};
  * angles: [L*n]; terms: the observable; ini_state: NULL (|0...0>) or a host
@@ -125,6 +148,21 @@ int qg_mcclean_shift_sample(qg_plan* plan, const int32_t* axes, const double* an
                             int nshift, const int32_t* shift_param, const double* shift_delta, const int32_t* order,
                             int shots, const double* u, int32_t* idx);
 
+/* ---- batched McClean: many parameter sets of one shape --------------------
+ * The reference runs one circuit per McClean.grad_run call (mc_clean.py:47-78);
+ * its callers that need many (an optimiser over several starting points,
+ * optimization.py:69-91) loop.  A batch plan holds up to `batch` circuits of n
+ * qubits and L layers, each with its own axes and angles, sharing one z / zz
+ * observable; every pass is one launch over all circuits resident at once
+ * (chunks sized to device memory).  Tile shape, memory mode (the per-pass
+ * history) and per-circuit grids are those of a single plan of the shape,
+ * so each circuit's result is bitwise what qg_mcclean_grad_run gives it. */
+int qg_mcclean_batch_plan(qg_ctx* ctx, int n, int L, int dtype, int entangler, int batch, qg_plan** out);
+/* axes int32 [B*L*n], angles [B*L*n] (circuit b at b*L*n); E_out[B],
+ * grad_out[B*L*n].  Observables with x / y terms: QG_ERR_UNSUPPORTED. */
+int qg_mcclean_batch_grad_run(qg_plan* plan, int B, const int32_t* axes, const double* angles, const qg_term* terms,
+                              int nterms, double* E_out, double* grad_out);
+
 /* ---- QAOA circuit (qradient/circuit_logic/qaoa.py) -----------------------
  * Plan for `batch` circuits of n qubits and p layers each. */
 int qg_qaoa_plan(qg_ctx* ctx, int n, int p, int dtype, int batch, qg_plan** out);
@@ -135,6 +173,15 @@ int qg_qaoa_plan(qg_ctx* ctx, int n, int p, int dtype, int batch, qg_plan** out)
  * grad_out[b*p*2 + i*2 + 0] = dE/dbeta_i, [.. + 1] = dE/dgamma_i. */
 int qg_qaoa_grad_run(qg_plan* plan, int B, const qg_term* terms, const int32_t* term_off, const double* betas,
                      const double* gammas, double* E_out, double* grad_out);
+
+/* Batch shape of a QAOA plan (no counterpart in the reference): kmax local
+ * bits per pass tile (0: the default rule), at least cmin contiguous low bits
+ * in high-bit passes, `chunk` circuits resident at once (0: as many as fit in
+ * device memory; fewer circuits per chunk keep more of a chunk's states in
+ * the 256 MB memory-side cache), and at most wg_per_cu workgroups per CU in
+ * a batched pass launch, over all circuits.  The plan is laid out again; used
+ * by the autotuner. */
+int qg_qaoa_tune(qg_plan* plan, int kmax, int cmin, int chunk, int wg_per_cu);
 
 /* Qaoa.run_expec_val with exact_expec_val=True (qaoa.py:23-38). */
 int qg_qaoa_expec_val(qg_plan* plan, int B, const qg_term* terms, const int32_t* term_off, const double* betas,
@@ -148,6 +195,18 @@ int qg_qaoa_grad_run_ini(qg_plan* plan, int B, const qg_term* terms, const int32
                          const double* gammas, const double* ini_states, double* E_out, double* grad_out);
 int qg_qaoa_expec_val_ini(qg_plan* plan, int B, const qg_term* terms, const int32_t* term_off, const double* betas,
                           const double* gammas, const double* ini_states, double* E_out);
+
+/* Lays a McClean plan out again for another circuit shape: n qubits, L
+ * layers, ckpt_every as for qg_mcclean_plan; dtype, entangler, context and
+ * tile options are kept.  The plan's small device tables only grow, so after
+ * it has held a shape at least as large in every respect (qubits, layers,
+ * passes) the call makes no device allocation.  A workspace supplied with
+ * qg_plan_set_workspace stays attached if it is large enough for the new
+ * layout and is detached otherwise (supply another, or the next run
+ * allocates one); an owned one that is too small is freed.  For serving
+ * loops whose requests change shape.  No counterpart in the reference (one
+ * circuit object per shape there). */
+int qg_plan_reshape(qg_plan* plan, int n, int L, int ckpt_every);
 
 int qg_plan_workspace_size(const qg_plan* plan, size_t* bytes);
 /* Hand the plan caller-owned device memory (e.g. a torch tensor) of at least
@@ -252,6 +311,64 @@ int qg_state_sample_basis(qg_state* st, const int32_t* order, int shots, const d
 /* dst.vec = src.vec on the device (same n and dtype): the reference's
  * state.vec[:] = state_history[k] (qaoa.py:139, 151) without a host copy. */
 int qg_state_copy(qg_state* dst, const qg_state* src);
+
+/* ---- serving: continuous batching of McClean grad runs ---------------------
+ * No counterpart in the reference, whose caller of grad_run is the optimiser
+ * loop (optimization.py:69-91), one circuit at a time.  A server owns one
+ * device arena of arena_bytes (allocated at creation) and `slots` worker
+ * threads, each with its own stream and one plan re-laid out per request
+ * (qg_plan_reshape), so requests of different shapes overlap on the device
+ * and a shape change allocates nothing once the plans have grown.  A request
+ * is admitted when a worker is idle and the arena can hold its workspace
+ * (qg_server_footprint); requests that fit may overtake a head-of-queue
+ * request that does not at most backfill_limit times.  A request's memory
+ * mode is a function of its shape only: the per-pass history when that
+ * needs at most history_max_bytes, else checkpoints every ckpt_every layers,
+ * so its result is bitwise the same alone and in any mix.  The observable is
+ * any set of terms; the entangler is the reference's CNOT ladder.  All
+ * server functions may be called from any thread. */
+#define QG_SERVER_SIMULATE 1 /* flags: no device; a request sleeps ~1 us per MiB of
+                                workspace and returns E = n, grad = L (scheduler tests) */
+typedef struct qg_server qg_server;
+typedef struct qg_request_info {
+  double queue_ms;        /* submit -> start on a worker (host clock) */
+  double run_ms;          /* start -> results ready */
+  double latency_ms;      /* submit -> results ready */
+  double device_start_ms; /* device time of the request's first launch, ms after server creation */
+  double device_end_ms;   /* device time after its last launch */
+  int32_t slot;           /* worker that ran it */
+  int32_t ckpt_every;     /* memory mode: 0 per-pass history, k > 0 checkpoints every k layers */
+  uint64_t footprint;     /* workspace bytes it held in the arena */
+} qg_request_info;
+typedef struct qg_server_counters {
+  int64_t queued, running, completed;
+  uint64_t arena_bytes, arena_in_use, arena_peak;
+} qg_server_counters;
+
+int qg_server_create(int device, int slots, int dtype, size_t arena_bytes, size_t history_max_bytes, int ckpt_every,
+                     int backfill_limit, int flags, qg_server** out);
+/* Workspace bytes a request of this shape holds in the arena and its memory
+ * mode (*ckpt_every = 0: history, k > 0: checkpoints; may be NULL). */
+int qg_server_footprint(qg_server* server, int n, int L, size_t* bytes, int* ckpt_every);
+/* Queue McClean.grad_run (mc_clean.py:47-78) of one circuit: axes int32 and
+ * angles [L*n], the observable terms.  The inputs are copied.  Invalid
+ * inputs fail here (QG_ERR_INVALID); a workspace larger than the arena fails
+ * with QG_ERR_NOMEM.  *ticket identifies the request. */
+int qg_submit(qg_server* server, int n, int L, const int32_t* axes, const double* angles, const qg_term* terms,
+              int nterms, int64_t* ticket);
+/* Waits up to timeout_ms (< 0: no limit) for the request; when it has
+ * finished, writes E, grad[L*n] and info (each may be NULL), forgets the
+ * ticket and returns the request's status; QG_ERR_PENDING if it is still
+ * queued or running. */
+int qg_poll(qg_server* server, int64_t ticket, int timeout_ms, double* E_out, double* grad_out,
+            qg_request_info* info);
+/* Waits up to timeout_ms (< 0: no limit) for any request to finish that no
+ * earlier qg_wait_any returned; *ticket receives it (collect it with
+ * qg_poll).  QG_ERR_PENDING on timeout. */
+int qg_wait_any(qg_server* server, int timeout_ms, int64_t* ticket);
+int qg_server_stats(qg_server* server, qg_server_counters* out);
+/* Runs every queued request to completion, then frees the server. */
+int qg_server_destroy(qg_server* server);
 
 #ifdef __cplusplus
 }

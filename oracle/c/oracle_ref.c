@@ -15,20 +15,31 @@
  *     the summed-X and -iC generators with tmp copies.
  * Gate formulas: state.py:90-97 (x), 142-149 (y), 168-175 (z); qubit q is
  * index bit n-1-q (state.py:84-87).  Single-threaded, complex128.
+ *
+ * Built with -DORACLE_REAL='long double' it runs the same algorithm in x87
+ * extended precision (64-bit mantissa), which pins the FP64 results: at the
+ * benchmark's depth two FP64 operation orders of the same algorithm differ by
+ * about 2e-12, and the extended-precision value tells which is closer.
+ * Inputs are the same doubles either way; outputs are rounded to double.
  */
 #include <complex.h>
-#include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <tgmath.h>
 
-typedef double complex cd;
+#ifndef ORACLE_REAL
+#define ORACLE_REAL double
+#endif
+typedef ORACLE_REAL real_t;
+typedef ORACLE_REAL _Complex cd; /* a typedef name cannot take _Complex */
 
 static void rot(cd* v, int n, int q, int axis, double a) {
   const uint64_t N = 1ull << n, bit = 1ull << (n - 1 - q);
-  const double c = cos(0.5 * a), s = sin(0.5 * a);
+  const real_t h = (real_t)a / 2;
+  const real_t c = cos(h), s = sin(h);
   if (axis == 2) {
-    const cd p0 = cexp(-0.5 * I * a), p1 = cexp(0.5 * I * a);
+    const cd p0 = cexp(-I * h), p1 = cexp(I * h);
     for (uint64_t x = 0; x < N; ++x) v[x] *= (x & bit) ? p1 : p0;
     return;
   }
@@ -47,9 +58,10 @@ static void rot(cd* v, int n, int q, int axis, double a) {
 
 static void drot(cd* v, int n, int q, int axis, double a) {
   const uint64_t N = 1ull << n, bit = 1ull << (n - 1 - q);
-  const double c = 0.5 * cos(0.5 * a), s = 0.5 * sin(0.5 * a);
+  const real_t h = (real_t)a / 2;
+  const real_t c = cos(h) / 2, s = sin(h) / 2;
   if (axis == 2) {
-    const cd p0 = -0.5 * I * cexp(-0.5 * I * a), p1 = 0.5 * I * cexp(0.5 * I * a);
+    const cd p0 = (-I / 2) * cexp(-I * h), p1 = (I / 2) * cexp(I * h);
     for (uint64_t x = 0; x < N; ++x) v[x] *= (x & bit) ? p1 : p0;
     return;
   }
@@ -90,11 +102,11 @@ static void apply_obs(const cd* v, cd* out, int n, int nt, const int* kind, cons
                       const double* w) {
   const uint64_t N = 1ull << n;
   for (uint64_t x = 0; x < N; ++x) {
-    double d = 0.0;
+    real_t d = 0.0;
     for (int k = 0; k < nt; ++k) {
       const int zi = (x >> (n - 1 - ii[k])) & 1 ? -1 : 1;
-      if (kind[k] == 2) d += w[k] * zi;
-      if (kind[k] == 3) d += w[k] * zi * (((x >> (n - 1 - jj[k])) & 1) ? -1 : 1);
+      if (kind[k] == 2) d += (real_t)w[k] * zi;
+      if (kind[k] == 3) d += (real_t)w[k] * zi * (((x >> (n - 1 - jj[k])) & 1) ? -1 : 1);
     }
     out[x] = d * v[x];
   }
@@ -103,14 +115,14 @@ static void apply_obs(const cd* v, cd* out, int n, int nt, const int* kind, cons
     const uint64_t bit = 1ull << (n - 1 - ii[k]);
     for (uint64_t x = 0; x < N; ++x) {
       const cd p = v[x ^ bit];
-      if (kind[k] == 0) out[x] += w[k] * p;
-      else out[x] += w[k] * ((x & bit) ? I * p : -I * p);
+      if (kind[k] == 0) out[x] += (real_t)w[k] * p;
+      else out[x] += (real_t)w[k] * ((x & bit) ? I * p : -I * p);
     }
   }
 }
 
-static double re_dot(const cd* a, const cd* b, uint64_t N) {
-  double s = 0.0;
+static real_t re_dot(const cd* a, const cd* b, uint64_t N) {
+  real_t s = 0.0;
   for (uint64_t x = 0; x < N; ++x) s += creal(conj(a[x]) * b[x]);
   return s;
 }
@@ -139,14 +151,14 @@ int oracle_mcclean_grad_run(int n, int L, int Ls, const int* axes, const double*
   }
   memcpy(hist + (size_t)Ls * N, v, sizeof(cd) * N);
   apply_obs(v, lam, n, nt, kind, ii, jj, w);
-  *E = re_dot(hist + (size_t)Ls * N, lam, N);
+  *E = (double)re_dot(hist + (size_t)Ls * N, lam, N);
   for (int i = Ls - 1; i >= 0; --i) {
     for (int q = 0; q < n; ++q) rot(lam, n, q, axes[i * n + q], -angles[i * n + q]);
     memcpy(tmp, lam, sizeof(cd) * N);
     for (int q = 0; q < n; ++q) {
       rot(lam, n, q, axes[i * n + q], angles[i * n + q]);
       drot(lam, n, q, axes[i * n + q], -angles[i * n + q]);
-      grad[i * n + q] = -2.0 * re_dot(hist + (size_t)i * N, lam, N);
+      grad[i * n + q] = (double)(-2 * re_dot(hist + (size_t)i * N, lam, N));
       memcpy(lam, tmp, sizeof(cd) * N);
     }
     ladder(lam, n, 1);
@@ -155,14 +167,14 @@ int oracle_mcclean_grad_run(int n, int L, int Ls, const int* axes, const double*
   return 0;
 }
 
-static void diag_C(double* C, int n, int nt, const int* kind, const int* ii, const int* jj, const double* w) {
+static void diag_C(real_t* C, int n, int nt, const int* kind, const int* ii, const int* jj, const double* w) {
   const uint64_t N = 1ull << n;
   for (uint64_t x = 0; x < N; ++x) {
-    double d = 0.0;
+    real_t d = 0.0;
     for (int k = 0; k < nt; ++k) {
       const int zi = (x >> (n - 1 - ii[k])) & 1 ? -1 : 1;
-      if (kind[k] == 2) d += w[k] * zi;
-      if (kind[k] == 3) d += w[k] * zi * (((x >> (n - 1 - jj[k])) & 1) ? -1 : 1);
+      if (kind[k] == 2) d += (real_t)w[k] * zi;
+      if (kind[k] == 3) d += (real_t)w[k] * zi * (((x >> (n - 1 - jj[k])) & 1) ? -1 : 1);
     }
     C[x] = d;
   }
@@ -176,22 +188,22 @@ int oracle_qaoa_grad_run(int n, int p, const double* betas, const double* gammas
   cd* v = (cd*)malloc(sizeof(cd) * N);
   cd* tmp = (cd*)malloc(sizeof(cd) * N);
   cd* u = (cd*)malloc(sizeof(cd) * N);
-  double* C = (double*)malloc(sizeof(double) * N);
+  real_t* C = (real_t*)malloc(sizeof(real_t) * N);
   if (!hist || !v || !tmp || !u || !C) {
     free(hist); free(v); free(tmp); free(u); free(C);
     return -1;
   }
   diag_C(C, n, nt, kind, ii, jj, w);
-  for (uint64_t x = 0; x < N; ++x) v[x] = pow(2.0, -0.5 * n);
+  for (uint64_t x = 0; x < N; ++x) v[x] = pow((real_t)2, -(real_t)n / 2);
   for (int i = 0; i < p; ++i) {
     memcpy(hist + (size_t)(2 * i) * N, v, sizeof(cd) * N);
-    for (uint64_t x = 0; x < N; ++x) v[x] *= cexp(-I * gammas[i] * C[x]);
+    for (uint64_t x = 0; x < N; ++x) v[x] *= cexp(-I * ((real_t)gammas[i] * C[x]));
     memcpy(hist + (size_t)(2 * i + 1) * N, v, sizeof(cd) * N);
     for (int q = 0; q < n; ++q) rot(v, n, q, 0, betas[i]);
   }
   memcpy(hist + (size_t)(2 * p) * N, v, sizeof(cd) * N);
   for (uint64_t x = 0; x < N; ++x) v[x] *= C[x];
-  *E = re_dot(hist + (size_t)(2 * p) * N, v, N);
+  *E = (double)re_dot(hist + (size_t)(2 * p) * N, v, N);
   for (int i = p - 1; i >= 0; --i) {
     for (int q = 0; q < n; ++q) rot(v, n, q, 0, -betas[i]);
     memcpy(tmp, v, sizeof(cd) * N);
@@ -199,14 +211,14 @@ int oracle_qaoa_grad_run(int n, int p, const double* betas, const double* gammas
     memset(u, 0, sizeof(cd) * N);
     for (int q = 0; q < n; ++q) {
       const uint64_t bit = 1ull << (n - 1 - q);
-      for (uint64_t x = 0; x < N; ++x) u[x] += 0.5 * (-I) * v[x ^ bit];
+      for (uint64_t x = 0; x < N; ++x) u[x] += (-I / 2) * v[x ^ bit];
     }
-    grad[2 * i] = -2.0 * re_dot(hist + (size_t)(2 * i + 1) * N, u, N);
+    grad[2 * i] = (double)(-2 * re_dot(hist + (size_t)(2 * i + 1) * N, u, N));
     memcpy(v, tmp, sizeof(cd) * N);
-    for (uint64_t x = 0; x < N; ++x) v[x] *= cexp(I * gammas[i] * C[x]);
+    for (uint64_t x = 0; x < N; ++x) v[x] *= cexp(I * ((real_t)gammas[i] * C[x]));
     memcpy(tmp, v, sizeof(cd) * N);
     for (uint64_t x = 0; x < N; ++x) u[x] = v[x] * (-I * C[x]);
-    grad[2 * i + 1] = -2.0 * re_dot(hist + (size_t)(2 * i) * N, u, N);
+    grad[2 * i + 1] = (double)(-2 * re_dot(hist + (size_t)(2 * i) * N, u, N));
     memcpy(v, tmp, sizeof(cd) * N);
   }
   free(hist); free(v); free(tmp); free(u); free(C);

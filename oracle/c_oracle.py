@@ -2,8 +2,10 @@
 
 TEST INFRASTRUCTURE ONLY: used by tests/ as a checker and by bench.py as the
 timed single-core CPU baseline.  Built with gcc by ``build()`` (called from
-``__graft_entry__.build()``); the shared object is git-ignored and travels
-with the working tree.
+``__graft_entry__.build()``) twice: in double precision (liboracle_ref.so) and
+in x87 extended precision (liboracle_ref_ld.so, ``extended=True``), the
+arbiter that pins FP64 results at depth.  The shared objects are git-ignored
+and travel with the working tree.
 """
 import ctypes
 import os
@@ -14,27 +16,30 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, 'c', 'oracle_ref.c')
 LIB = os.path.join(HERE, 'c', 'liboracle_ref.so')
-_lib = None
+LIB_LD = os.path.join(HERE, 'c', 'liboracle_ref_ld.so')
+_libs = {}
 
 
 def build():
-    cmd = ['gcc', '-O3', '-march=x86-64-v2', '-fPIC', '-shared', '-std=gnu11', SRC, '-o', LIB + '.tmp', '-lm']
-    subprocess.run(cmd, check=True, capture_output=True)
-    os.replace(LIB + '.tmp', LIB)
+    for out, extra in ((LIB, []), (LIB_LD, ['-DORACLE_REAL=long double'])):
+        cmd = ['gcc', '-O3', '-march=x86-64-v2', '-fPIC', '-shared', '-std=gnu11'] + extra + [SRC, '-o', out + '.tmp',
+                                                                                             '-lm']
+        subprocess.run(cmd, check=True, capture_output=True)
+        os.replace(out + '.tmp', out)
     return LIB
 
 
-def load():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
+def load(extended=False):
+    path = LIB_LD if extended else LIB
+    if path not in _libs:
+        if not os.path.exists(path):
             build()
-        lib = ctypes.CDLL(LIB)
+        lib = ctypes.CDLL(path)
         ip, dp = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
         lib.oracle_mcclean_grad_run.argtypes = [ctypes.c_int] * 3 + [ip, dp, ctypes.c_int, ip, ip, ip, dp, dp, dp]
         lib.oracle_qaoa_grad_run.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_int, ip, ip, ip, dp, dp, dp]
-        _lib = lib
-    return _lib
+        _libs[path] = lib
+    return _libs[path]
 
 
 def _terms(obs):
@@ -52,9 +57,10 @@ def _p(a, t):
     return a.ctypes.data_as(ctypes.POINTER(t))
 
 
-def mcclean_grad_run(n, axes, angles, obs, layers=None):
-    """(E, grad) of the reference algorithm; ``layers`` < L runs a bounded sample."""
-    lib = load()
+def mcclean_grad_run(n, axes, angles, obs, layers=None, extended=False):
+    """(E, grad) of the reference algorithm; ``layers`` < L runs a bounded sample;
+    ``extended``: in x87 extended precision."""
+    lib = load(extended)
     L = axes.shape[0]
     Ls = L if layers is None else int(layers)
     ax = np.ascontiguousarray(axes, dtype=np.int32)
@@ -70,8 +76,8 @@ def mcclean_grad_run(n, axes, angles, obs, layers=None):
     return float(E[0]), grad
 
 
-def qaoa_grad_run(n, obs, betas, gammas):
-    lib = load()
+def qaoa_grad_run(n, obs, betas, gammas, extended=False):
+    lib = load(extended)
     b = np.ascontiguousarray(betas, dtype=np.float64)
     g = np.ascontiguousarray(gammas, dtype=np.float64)
     nt, kind, ii, jj, w = _terms(obs)

@@ -12,7 +12,8 @@ import numpy as np
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'libqradient_amd.so')
 
-QG_OK, QG_ERR_INVALID, QG_ERR_UNSUPPORTED, QG_ERR_HIP, QG_ERR_NOMEM = 0, 1, 2, 3, 4
+QG_OK, QG_ERR_INVALID, QG_ERR_UNSUPPORTED, QG_ERR_HIP, QG_ERR_NOMEM, QG_ERR_PENDING = 0, 1, 2, 3, 4, 5
+QG_SERVER_SIMULATE = 1
 QG_C128, QG_C64 = 0, 1
 QG_ENT_CNOT, QG_ENT_CZ = 0, 1
 QG_HISTORY = -1
@@ -26,6 +27,17 @@ ENTANGLERS = {'cnot': QG_ENT_CNOT, 'cz': QG_ENT_CZ}
 class QgTerm(ctypes.Structure):
     _fields_ = [('kind', ctypes.c_int32), ('i', ctypes.c_int32), ('j', ctypes.c_int32),
                 ('pad', ctypes.c_int32), ('w', ctypes.c_double)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [('queue_ms', ctypes.c_double), ('run_ms', ctypes.c_double), ('latency_ms', ctypes.c_double),
+                ('device_start_ms', ctypes.c_double), ('device_end_ms', ctypes.c_double),
+                ('slot', ctypes.c_int32), ('ckpt_every', ctypes.c_int32), ('footprint', ctypes.c_uint64)]
+
+
+class ServerCounters(ctypes.Structure):
+    _fields_ = [('queued', ctypes.c_int64), ('running', ctypes.c_int64), ('completed', ctypes.c_int64),
+                ('arena_bytes', ctypes.c_uint64), ('arena_in_use', ctypes.c_uint64), ('arena_peak', ctypes.c_uint64)]
 
 
 _vp = ctypes.c_void_p
@@ -43,18 +55,24 @@ SIGNATURES = {
     'qg_ctx_create': [_i, _vp, _pp],
     'qg_ctx_destroy': [_vp],
     'qg_ctx_synchronize': [_vp],
+    'qg_stream_copy': [_vp, _vp, _vp, ctypes.c_size_t, _i, _i, _i, _dp],
     'qg_mcclean_plan': [_vp, _i, _i, _i, _i, _i, _pp],
     'qg_mcclean_grad_run': [_vp, _ip, _dp, _tp, _i, _dp, _dp, _dp],
+    'qg_mcclean_workspace_bytes': [_vp, _i, _i, _i, _i, ctypes.POINTER(ctypes.c_size_t), _ip],
     'qg_mcclean_expec_val': [_vp, _ip, _dp, _tp, _i, _dp, _dp],
     'qg_mcclean_shift_term_prob': [_vp, _ip, _dp, _dp, _i, _ip, _dp, _tp, _i, _dp],
     'qg_mcclean_shift_sample': [_vp, _ip, _dp, _dp, _i, _ip, _dp, _ip, _i, _dp, _ip],
     'qg_state_sample_basis': [_vp, _ip, _i, _dp, _ip],
     'qg_state_copy': [_vp, _vp],
+    'qg_mcclean_batch_plan': [_vp, _i, _i, _i, _i, _i, _pp],
+    'qg_mcclean_batch_grad_run': [_vp, _i, _ip, _dp, _tp, _i, _dp, _dp],
     'qg_qaoa_plan': [_vp, _i, _i, _i, _i, _pp],
+    'qg_qaoa_tune': [_vp, _i, _i, _i, _i],
     'qg_qaoa_grad_run': [_vp, _i, _tp, _ip, _dp, _dp, _dp, _dp],
     'qg_qaoa_expec_val': [_vp, _i, _tp, _ip, _dp, _dp, _dp],
     'qg_qaoa_grad_run_ini': [_vp, _i, _tp, _ip, _dp, _dp, _dp, _dp, _dp],
     'qg_qaoa_expec_val_ini': [_vp, _i, _tp, _ip, _dp, _dp, _dp, _dp],
+    'qg_plan_reshape': [_vp, _i, _i, _i],
     'qg_plan_workspace_size': [_vp, ctypes.POINTER(ctypes.c_size_t)],
     'qg_plan_set_workspace': [_vp, _vp, ctypes.c_size_t],
     'qg_plan_get_state': [_vp, _dp],
@@ -86,6 +104,13 @@ SIGNATURES = {
     'qg_plan_term_prob': [_vp, _tp, _i, _dp],
     'qg_plan_expec_val': [_vp, _tp, _i, _dp],
     'qg_state_scale': [_vp, _d, _d],
+    'qg_server_create': [_i, _i, _i, ctypes.c_size_t, ctypes.c_size_t, _i, _i, _i, _pp],
+    'qg_server_footprint': [_vp, _i, _i, ctypes.POINTER(ctypes.c_size_t), _ip],
+    'qg_submit': [_vp, _i, _i, _ip, _dp, _tp, _i, ctypes.POINTER(ctypes.c_int64)],
+    'qg_poll': [_vp, ctypes.c_int64, _i, _dp, _dp, ctypes.POINTER(RequestInfo)],
+    'qg_wait_any': [_vp, _i, ctypes.POINTER(ctypes.c_int64)],
+    'qg_server_stats': [_vp, ctypes.POINTER(ServerCounters)],
+    'qg_server_destroy': [_vp],
 }
 _RESTYPES = {'qg_last_error': ctypes.c_char_p}
 

@@ -8,9 +8,9 @@ grad[i, q] = dE/d angles[i, q], computed by the adjoint sweep in HIP kernels.
 Backend keywords (not in the reference): ``dtype='complex128'`` (the plain
 method, FP64 throughout) or ``'complex64'`` (complex64 state, FP64 arithmetic
 and reductions); ``entangler='cnot'`` (reference) or ``'cz'``; ``device``;
-``ckpt_every``; ``tune=True`` (apply the autotuner's cached tile shape for
-this circuit shape, timing the candidates on the device first if none is
-cached); ``graph=True`` (replay the pass launches as a HIP graph; for callers
+``ckpt_every``; ``tune=True`` (apply the autotuner's cached tile shape and,
+for circuits whose per-pass history does not fit, checkpoint period for this
+circuit shape, timing the candidates on the device first if none is cached); ``graph=True`` (replay the pass launches as a HIP graph; for callers
 that make no concurrent device-synchronising HIP calls from other threads).
 """
 import ctypes
@@ -42,9 +42,19 @@ class McClean(ParametrizedCircuit):
 
     def _get_plan(self):
         if self._plan is None:
+            ckpt = self.ckpt_every
+            if self.tune and ckpt == 0:
+                # the tuned checkpoint period of this shape, when its history does not fit
+                from .. import autotune
+                k = autotune.lookup(autotune.ckpt_key(self.qnum, self.lnum, self.dtype, self.entangler))
+                if k is None:
+                    cand = autotune.ckpt_candidates(self.context(), self.qnum, self.lnum, self.dtype)
+                    k = cand[0] if cand == [0] else autotune.tune_ckpt(
+                        self.qnum, self.lnum, self.dtype, self.entangler, device=self.device)[0]
+                ckpt = int(k)
             h = ctypes.c_void_p()
             _native.call('qg_mcclean_plan', self.context().handle, self.qnum, self.lnum,
-                         _native.DTYPES[self.dtype], _native.ENTANGLERS[self.entangler], self.ckpt_every,
+                         _native.DTYPES[self.dtype], _native.ENTANGLERS[self.entangler], ckpt,
                          ctypes.byref(h))
             self._plan = h
             if self.graph:

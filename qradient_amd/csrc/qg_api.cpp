@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <limits>
 #include <vector>
 
 #include "../../include/qradient_amd.h"
@@ -39,6 +40,7 @@ constexpr int kDotBlocks = 512;
 hipError_t launch_fused(const void* descs, int batch, int n, hipStream_t st);
 hipError_t set_fused_attributes();
 size_t fused_desc_size();
+hipError_t launch_stream_copy(void* dst, const void* src, size_t bytes, bool nontemporal, int grid, hipStream_t st);
 
 // Mirror of the device-side FusedDesc in qg_fused.hip (same layout).
 struct FusedDescH {
@@ -86,6 +88,11 @@ static int fail(int code, const std::string& msg) {
     if (r_ != QG_OK) return r_;    \
   } while (0)
 
+namespace qg {
+// The calling thread's qg_last_error, for the other host translation units.
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace qg
+
 extern "C" const char* qg_last_error(void) { return g_err.c_str(); }
 extern "C" int qg_version(void) { return 1; }
 
@@ -98,6 +105,7 @@ struct qg_ctx {
   bool own_stream = false;
   int num_cu = 256;
   size_t max_lds = 65536;
+  size_t total_mem = 0;  // device memory, queried once (plan layouts are sized against it)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -115,6 +123,10 @@ extern "C" int qg_ctx_create(int device, void* stream, qg_ctx** out) {
   c->device = device;
   c->num_cu = prop.multiProcessorCount;
   c->max_lds = prop.sharedMemPerBlock;
+  {
+    size_t free_b = 0;
+    HIPCHK(hipMemGetInfo(&free_b, &c->total_mem));
+  }
   if (stream) {
     c->stream = (hipStream_t)stream;
   } else {
@@ -143,6 +155,25 @@ extern "C" int qg_ctx_destroy(qg_ctx* c) {
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
+  return QG_OK;
+}
+
+extern "C" int qg_stream_copy(qg_ctx* c, void* dst, const void* src, size_t bytes, int nontemporal,
+                              int grid_per_cu, int reps, double* ms_out) {
+  if (!c || !dst || !src || !ms_out) return fail(QG_ERR_INVALID, "NULL argument");
+  if (bytes == 0 || bytes % 16 != 0 || ((uintptr_t)dst | (uintptr_t)src) % 16 != 0)
+    return fail(QG_ERR_INVALID, "qg_stream_copy: bytes and both pointers must be multiples of 16");
+  if (reps < 1 || grid_per_cu < 1 || grid_per_cu > 64)
+    return fail(QG_ERR_INVALID, "qg_stream_copy: reps >= 1 and grid_per_cu in [1, 64]");
+  HIPCHK(hipSetDevice(c->device));
+  const int grid = c->num_cu * grid_per_cu;
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  for (int r = 0; r < reps; ++r) HIPCHK(launch_stream_copy(dst, src, bytes, nontemporal != 0, grid, c->stream));
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  HIPCHK(hipEventSynchronize(c->ev1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  *ms_out = (double)ms / reps;
   return QG_OK;
 }
 
@@ -275,8 +306,7 @@ static int local_of_global(const Geo& g, int b) {  // -1 if not local
 
 // Columns of the basis-index map of CNOT-ladder stacking `s` (state.py:229-241):
 // stacking 1 acts the even bonds first, stacking 0 the odd bonds first.
-static void ladder_map(int n, int stacking, std::vector<uint32_t>& cols) {
-  cols.assign(n, 0);
+static void ladder_map(int n, int stacking, uint32_t* cols) {
   for (int k = 0; k < n; ++k) {
     uint32_t v = 1u << k;
     for (int grp = 0; grp < 2; ++grp) {
@@ -286,6 +316,18 @@ static void ladder_map(int n, int stacking, std::vector<uint32_t>& cols) {
     }
     cols[k] = v;
   }
+}
+
+static void ladder_map(int n, int stacking, std::vector<uint32_t>& cols) {
+  cols.assign(n, 0);
+  ladder_map(n, stacking, cols.data());
+}
+
+// Amplitude of |x> in the RY(pi/4) product state by popcount k = |x|
+// (mc_clean.py:55-56): cos(pi/8)^(n-k) sin(pi/8)^k, for k = 0..n.
+static void ry_init_table(int n, double* tab) {
+  const double c8 = std::cos(M_PI / 8), s8 = std::sin(M_PI / 8);
+  for (int k = 0; k <= n; ++k) tab[k] = std::pow(c8, n - k) * std::pow(s8, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -318,7 +360,11 @@ struct qg_plan {
   int r_fwd = 4, r_rev = 4;  // amplitudes per thread (log2) of forward / history-adjoint passes (complex64)
   int lds64_req = -1;  // complex64 McClean: FP64 LDS tiles (-1 auto, 0 no, 1 yes)
   bool lds64 = false;
-  int chunk = 1;  // circuits resident at once (QAOA batches)
+  int chunk = 1;  // circuits resident at once (QAOA and McClean batches)
+  int chunk_req = 0;  // QAOA: requested chunk (0: as many circuits as fit)
+  int wg_per_cu = 16; // QAOA: workgroups per CU of a batched pass launch, in all circuits
+  size_t qaoa_terms_cap = 0;  // QAOA: DTerm capacity of d_terms (chunk x terms per circuit)
+  int mc_batch = 0;  // McClean batch plans (qg_mcclean_batch_plan): circuits per call
   bool fused = true;       // n <= 13: whole circuit per workgroup (qg_fused.hip)
   char* h_stage = nullptr;  // pinned staging of QAOA term and phase tables
   size_t stage_cap = 0;
@@ -334,7 +380,8 @@ struct qg_plan {
   size_t S = 0;  // bytes per state
   // workspace
   char* ws = nullptr;
-  size_t ws_bytes = 0;
+  size_t ws_bytes = 0;  // what the current layout needs
+  size_t ws_cap = 0;    // size of the attached workspace (allocated or supplied)
   bool ws_owned = false;
   size_t off_hist = 0, off_lam = 0, off_ini = 0, off_prep = 0;
   // small device tables
@@ -345,6 +392,7 @@ struct qg_plan {
   std::vector<DTerm> h_cterms;  // the terms split per pass launch (PassDesc::cterms)
   DTerm* d_cterms = nullptr;
   void* d_ini_states = nullptr;  // QAOA: the chunk's initial states (when given)
+  size_t ini_cap = 0;            // its size in bytes
   size_t cterm_cap = 0;
   DTerm* d_xy = nullptr;
   int max_terms = 0;
@@ -354,6 +402,14 @@ struct qg_plan {
   int gmax = 1, nrows = 0;
   int* d_fin = nullptr;         // ptr | rows | cnt
   size_t fin_cap = 0;
+  // capacities of the small tables (they only grow, so a plan re-laid out for
+  // a shape no larger than one it held before allocates nothing)
+  int cap_n = 0;            // d_map (n), d_init (n + 1), h_tabs
+  int cap_terms = 0;        // d_terms, d_xy
+  size_t cap_part = 0;      // d_part elements (rows x gmax)
+  int cap_nout = 0;         // d_scale, d_out, h_out
+  size_t cap_fstate = 0;    // d_fstate bytes
+  char* h_tabs = nullptr;   // pinned staging of the map and RY(pi/4) tables
   double* d_scale = nullptr;
   double* d_out = nullptr;
   double* d_dot = nullptr;      // kDotBlocks + 1
@@ -406,6 +462,7 @@ static int ensure_workspace(qg_plan* p) {
     return fail(QG_ERR_NOMEM, "workspace allocation of " + std::to_string(p->ws_bytes) + " bytes failed: " +
                                   hipGetErrorString(e));
   p->ws = (char*)m;
+  p->ws_cap = p->ws_bytes;
   p->ws_owned = true;
   return QG_OK;
 }
@@ -476,24 +533,87 @@ static int grid_max(const qg_plan* p) {
   return gm;
 }
 
-static int alloc_small(qg_plan* p, int ndesc, int nrows, int nout) {
+// Grows the small device tables of a plan to hold ndesc descriptors, nrows
+// partial rows of gmax workgroups and nout results (and the terms and static
+// tables of an n-qubit circuit); tables already large enough are kept.
+static int ensure_small(qg_plan* p, int ndesc, int nrows, int nout) {
   const int n = p->n;
-  HIPCHK(hipMalloc(&p->d_map, sizeof(uint32_t) * std::max(n, 1)));
-  HIPCHK(hipMalloc(&p->d_init, sizeof(double) * (n + 1)));
-  p->max_terms = std::max(64, n * n);
-  HIPCHK(hipMalloc(&p->d_terms, sizeof(DTerm) * p->max_terms));
-  HIPCHK(hipMalloc(&p->d_xy, sizeof(DTerm) * p->max_terms));
-  HIPCHK(hipMalloc(&p->d_part, sizeof(double) * (size_t)std::max(1, nrows) * p->gmax));
-  HIPCHK(hipMemset(p->d_part, 0, sizeof(double) * (size_t)std::max(1, nrows) * p->gmax));
-  p->fin_cap = (size_t)(3 * (nout + 1) + 3 * nrows + 8);
-  HIPCHK(hipMalloc(&p->d_fin, sizeof(int) * p->fin_cap));
-  HIPCHK(hipMalloc(&p->d_scale, sizeof(double) * std::max(1, nout)));
-  HIPCHK(hipMalloc(&p->d_out, sizeof(double) * std::max(1, nout)));
-  HIPCHK(hipMalloc(&p->d_dot, sizeof(double) * (kDotBlocks + 1)));
-  HIPCHK(hipMalloc(&p->d_desc, sizeof(PassDesc) * std::max(1, ndesc)));
-  HIPCHK(hipHostMalloc(&p->h_desc, sizeof(PassDesc) * std::max(1, ndesc), hipHostMallocDefault));
-  HIPCHK(hipHostMalloc(&p->h_out, sizeof(double) * std::max(1, nout), hipHostMallocDefault));
-  p->ndesc_cap = ndesc;
+  const bool grow_n = n > p->cap_n || !p->d_map;
+  const bool grow_terms = std::max(64, n * n) > p->cap_terms || !p->d_terms;
+  const size_t part = (size_t)std::max(1, nrows) * p->gmax;
+  const size_t fin = (size_t)(3 * (nout + 1) + 3 * nrows + 8);
+  const bool grow_part = part > p->cap_part || !p->d_part;
+  const bool grow_out = nout > p->cap_nout || !p->d_out;
+  const bool grow_fin = fin > p->fin_cap || !p->d_fin;
+  const bool grow_desc = ndesc > p->ndesc_cap || !p->d_desc;
+  if (grow_n || grow_terms || grow_part || grow_out || grow_fin || grow_desc) {
+    // the tables being replaced may still be read by queued work
+    HIPCHK(hipStreamSynchronize(p->ctx->stream));
+    drop_graph(p);  // a captured graph holds their addresses
+  }
+  if (grow_n) {
+    (void)hipFree(p->d_map);
+    (void)hipFree(p->d_init);
+    if (p->h_tabs) (void)hipHostFree(p->h_tabs);
+    p->d_map = nullptr;
+    p->d_init = nullptr;
+    p->h_tabs = nullptr;
+    p->cap_n = 0;
+    HIPCHK(hipMalloc(&p->d_map, sizeof(uint32_t) * std::max(n, 1)));
+    HIPCHK(hipMalloc(&p->d_init, sizeof(double) * (n + 1)));
+    HIPCHK(hipHostMalloc(&p->h_tabs, sizeof(uint32_t) * std::max(n, 1) + sizeof(double) * (n + 1) + 16,
+                         hipHostMallocDefault));
+    p->cap_n = n;
+  }
+  if (grow_terms) {
+    (void)hipFree(p->d_terms);
+    (void)hipFree(p->d_xy);
+    p->d_terms = p->d_xy = nullptr;
+    p->cap_terms = 0;
+    const int mt = std::max(64, n * n);
+    HIPCHK(hipMalloc(&p->d_terms, sizeof(DTerm) * mt));
+    HIPCHK(hipMalloc(&p->d_xy, sizeof(DTerm) * mt));
+    p->cap_terms = mt;
+  }
+  p->max_terms = p->cap_terms;
+  if (grow_part) {
+    (void)hipFree(p->d_part);
+    p->d_part = nullptr;
+    p->cap_part = 0;
+    HIPCHK(hipMalloc(&p->d_part, sizeof(double) * part));
+    HIPCHK(hipMemset(p->d_part, 0, sizeof(double) * part));
+    p->cap_part = part;
+  }
+  if (grow_fin) {
+    (void)hipFree(p->d_fin);
+    p->d_fin = nullptr;
+    p->fin_cap = 0;
+    HIPCHK(hipMalloc(&p->d_fin, sizeof(int) * fin));
+    p->fin_cap = fin;
+  }
+  if (grow_out) {
+    (void)hipFree(p->d_scale);
+    (void)hipFree(p->d_out);
+    if (p->h_out) (void)hipHostFree(p->h_out);
+    p->d_scale = p->d_out = nullptr;
+    p->h_out = nullptr;
+    p->cap_nout = 0;
+    HIPCHK(hipMalloc(&p->d_scale, sizeof(double) * std::max(1, nout)));
+    HIPCHK(hipMalloc(&p->d_out, sizeof(double) * std::max(1, nout)));
+    HIPCHK(hipHostMalloc(&p->h_out, sizeof(double) * std::max(1, nout), hipHostMallocDefault));
+    p->cap_nout = nout;
+  }
+  if (!p->d_dot) HIPCHK(hipMalloc(&p->d_dot, sizeof(double) * (kDotBlocks + 1)));
+  if (grow_desc) {
+    (void)hipFree(p->d_desc);
+    if (p->h_desc) (void)hipHostFree(p->h_desc);
+    p->d_desc = nullptr;
+    p->h_desc = nullptr;
+    p->ndesc_cap = 0;
+    HIPCHK(hipMalloc(&p->d_desc, sizeof(PassDesc) * std::max(1, ndesc)));
+    HIPCHK(hipHostMalloc(&p->h_desc, sizeof(PassDesc) * std::max(1, ndesc), hipHostMallocDefault));
+    p->ndesc_cap = ndesc;
+  }
   p->nout = nout;
   return QG_OK;
 }
@@ -526,6 +646,7 @@ extern "C" int qg_plan_set_workspace(qg_plan* p, void* dev, size_t bytes) {
     (void)hipFree(p->ws);
   }
   p->ws = (char*)dev;
+  p->ws_cap = bytes;
   p->ws_owned = false;
   return QG_OK;
 }
@@ -927,9 +1048,10 @@ static int run_launches(qg_plan* p, int ndesc, int first_desc = 0) {
   const size_t nl = p->launches.size();
   std::vector<hipEvent_t>* pev = nullptr;
   if (p->profile) {
-    if (p->prof_runs >= kMaxProfRuns) p->prof_runs = 0;  // start over: only the latest runs are kept
-    if (p->prof_ev.size() <= p->prof_runs) p->prof_ev.resize(p->prof_runs + 1);
-    pev = &p->prof_ev[p->prof_runs];
+    // a ring of the latest kMaxProfRuns runs: run r of the series uses slot r mod kMaxProfRuns
+    const size_t slot = p->prof_runs % kMaxProfRuns;
+    if (p->prof_ev.size() <= slot) p->prof_ev.resize(slot + 1);
+    pev = &p->prof_ev[slot];
     while (pev->size() < nl + 1) {
       hipEvent_t e;
       HIPCHK(hipEventCreate(&e));
@@ -1039,8 +1161,9 @@ static int stream_flush(qg_plan* p) {
 static int collect_profile(qg_plan* p) {
   if (!p->profile) return QG_OK;
   const size_t nl = p->launches.size();
-  if (p->prof_kind.size() <= p->prof_runs) p->prof_kind.resize(p->prof_runs + 1);
-  std::vector<int>& kinds = p->prof_kind[p->prof_runs];
+  const size_t slot = p->prof_runs % kMaxProfRuns;
+  if (p->prof_kind.size() <= slot) p->prof_kind.resize(slot + 1);
+  std::vector<int>& kinds = p->prof_kind[slot];
   kinds.assign(nl, 0);
   for (size_t k = 0; k < nl; ++k) {
     if (p->launches[k].kind != 0) {
@@ -1069,9 +1192,10 @@ extern "C" int qg_plan_pass_times(const qg_plan* p, int cap, float* ms, int* kin
     HIPCHK(hipSetDevice(p->ctx->device));
     p->pass_ms.clear();
     p->pass_kind.clear();
-    for (size_t r = 0; r < p->prof_runs; ++r) {
-      const std::vector<hipEvent_t>& ev = p->prof_ev[r];
-      const std::vector<int>& kd = p->prof_kind[r];
+    const size_t first = p->prof_runs > kMaxProfRuns ? p->prof_runs - kMaxProfRuns : 0;
+    for (size_t r = first; r < p->prof_runs; ++r) {  // oldest kept run first
+      const std::vector<hipEvent_t>& ev = p->prof_ev[r % kMaxProfRuns];
+      const std::vector<int>& kd = p->prof_kind[r % kMaxProfRuns];
       HIPCHK(hipEventSynchronize(ev[kd.size()]));
       for (size_t k = 0; k < kd.size(); ++k) {
         float t = 0.f;
@@ -1099,6 +1223,7 @@ static void free_small(qg_plan* p) {
   p->d_cterms = nullptr;
   (void)hipFree(p->d_ini_states);
   p->d_ini_states = nullptr;
+  p->ini_cap = 0;
   p->cterm_cap = 0;
   (void)hipFree(p->d_map);
   (void)hipFree(p->d_init);
@@ -1136,23 +1261,40 @@ static void free_small(qg_plan* p) {
   p->h_out = nullptr;
   p->lut_len = 0;
   p->fin_cap = 0;
+  if (p->h_tabs) (void)hipHostFree(p->h_tabs);
+  p->h_tabs = nullptr;
+  p->cap_n = p->cap_terms = p->cap_nout = p->ndesc_cap = 0;
+  p->cap_part = p->cap_fstate = 0;
 }
 
-// Geometry, memory mode (history or checkpointed uncompute), workspace layout
-// and the small device tables of a McClean plan.
-static int mcclean_layout(qg_plan* p) {
+// Memory a plan's layout may assume: this fraction of the device's total
+// memory.  The layout (history or checkpoints, the period, the tile size) is a
+// function of the circuit shape and the device alone, never of what happens
+// to be free, so one shape always gives the same numerics; a workspace that
+// cannot then be allocated is an error (QG_ERR_NOMEM), not a different mode.
+constexpr double kPlanMemFraction = 0.75;
+
+static int plan_budget(qg_plan* p, size_t* budget) {
+  *budget = (size_t)(kPlanMemFraction * (double)p->ctx->total_mem);
+  return QG_OK;
+}
+
+// Geometry, memory mode (history or checkpointed uncompute) and workspace
+// layout of a McClean plan: a function of the shape, the tile options and the
+// device's total memory only (no device call), so the footprint of a shape
+// can be computed without a plan.
+static int mcclean_geometry(qg_plan* p) {
   const int n = p->n, L = p->L;
   if (p->kmax_auto) p->kmax = default_kmax(n, p->dtype, p->cmin);
   p->geo = layer_geometry(n, p->kmax, p->cmin);
   int P = (int)p->geo.size();
+  size_t budget = 0;
+  CHK(plan_budget(p, &budget));
   if (p->kmax > 13) {
     // 14 local bits run only as complex64 history passes with FP32 tiles
     // (1024-thread variants exist for those); otherwise fall back to 13
     const size_t S = (size_t)(1ull << n) * 8;
-    size_t fb = 0, tb = 0;
-    HIPCHK(hipMemGetInfo(&fb, &tb));
-    if (p->ws_owned && p->ws) fb += p->ws_bytes;
-    const bool hist_fits = p->ckpt_req <= 0 && (size_t)(P * L + 4) * S <= (size_t)(0.85 * (double)fb);
+    const bool hist_fits = p->ckpt_req <= 0 && (size_t)(P * L + 4) * S <= budget;
     const bool fp64_tiles = p->lds64_req == 1 || (p->lds64_req < 0 && P * L > 200);
     if (p->dtype != QG_C64 || !hist_fits || fp64_tiles) {
       p->kmax = p->kmax_auto ? default_kmax(n, p->dtype, p->cmin, false) : 13;
@@ -1166,10 +1308,6 @@ static int mcclean_layout(qg_plan* p) {
   // past a 1e-6 relative error, so long circuits use FP64 tiles (rounding
   // once per pass, at the store)
   p->lds64 = p->dtype == QG_C64 && n >= 4 && (p->lds64_req == 1 || (p->lds64_req < 0 && P * L > 200));
-  size_t free_b = 0, total_b = 0;
-  HIPCHK(hipMemGetInfo(&free_b, &total_b));
-  if (p->ws_owned && p->ws) free_b += p->ws_bytes;
-  const size_t budget = (size_t)(0.85 * (double)free_b);
   const size_t hist_bytes = (size_t)(P * L + 4) * p->S;
   if (p->ckpt_req == QG_HISTORY && hist_bytes > budget)
     return fail(QG_ERR_NOMEM, "the per-pass history does not fit in device memory");
@@ -1201,18 +1339,27 @@ static int mcclean_layout(qg_plan* p) {
   p->off_ini = p->off_lam + 2 * p->S;
   p->off_prep = p->off_ini + p->S;
   p->ws_bytes = p->off_prep + p->S;
+  return QG_OK;
+}
+
+// mcclean_geometry plus the partial-row stride and the small device tables.
+static int mcclean_layout(qg_plan* p) {
+  CHK(mcclean_geometry(p));
+  const int n = p->n, L = p->L, P = (int)p->geo.size();
   p->gmax = grid_max(p);
   const int npass = 2 * P * L + 2 * P + 1;  // forward + reverse + ini-state preparation + observable
   p->nrows = (P * L + 1) * (kMaxK + 1);
-  CHK(alloc_small(p, npass, p->nrows, 1 + L * n));
-  // static tables
-  std::vector<uint32_t> map;
-  ladder_map(n, 1, map);
-  std::vector<double> init(n + 1);
-  const double c8 = std::cos(M_PI / 8), s8 = std::sin(M_PI / 8);
-  for (int k = 0; k <= n; ++k) init[k] = std::pow(c8, n - k) * std::pow(s8, k);
-  HIPCHK(hipMemcpy(p->d_map, map.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(p->d_init, init.data(), sizeof(double) * (n + 1), hipMemcpyHostToDevice));
+  CHK(ensure_small(p, npass, p->nrows, 1 + L * n));
+  // static tables, staged in pinned memory and copied in stream order (the
+  // previous run on this stream ended with a synchronisation, so the staging
+  // is free and the tables are no longer read)
+  uint32_t* hm = (uint32_t*)p->h_tabs;
+  double* hi = (double*)(p->h_tabs + ((sizeof(uint32_t) * std::max(n, 1) + 15) & ~(size_t)15));
+  ladder_map(n, 1, hm);
+  ry_init_table(n, hi);
+  HIPCHK(hipMemcpyAsync(p->d_map, hm, sizeof(uint32_t) * n, hipMemcpyHostToDevice, p->ctx->stream));
+  HIPCHK(hipMemcpyAsync(p->d_init, hi, sizeof(double) * (n + 1), hipMemcpyHostToDevice, p->ctx->stream));
+  HIPCHK(hipStreamSynchronize(p->ctx->stream));
   return QG_OK;
 }
 
@@ -1239,6 +1386,30 @@ extern "C" int qg_mcclean_plan(qg_ctx* ctx, int n, int L, int dtype, int entangl
     return r;
   }
   *out = p;
+  return QG_OK;
+}
+
+extern "C" int qg_mcclean_workspace_bytes(qg_ctx* ctx, int n, int L, int dtype, int ckpt_every, size_t* bytes,
+                                          int* ckpt_chosen) {
+  if (!bytes) return fail(QG_ERR_INVALID, "NULL argument");
+  if (n < 1 || n > kMaxQubits) return fail(QG_ERR_INVALID, "qubit number must be in [1, 32]");
+  if (L < 1) return fail(QG_ERR_INVALID, "layer number must be >= 1");
+  if (dtype != QG_C128 && dtype != QG_C64) return fail(QG_ERR_INVALID, "dtype must be QG_C128 or QG_C64");
+  if (ckpt_every < QG_HISTORY) return fail(QG_ERR_INVALID, "ckpt_every must be >= 0 or QG_HISTORY");
+  if (!ctx && ckpt_every == 0)
+    return fail(QG_ERR_INVALID, "without a context the memory mode must be given (QG_HISTORY or k > 0)");
+  qg_ctx none;
+  none.total_mem = std::numeric_limits<size_t>::max() / 2;  // no device: no memory bound
+  qg_plan p;
+  p.ctx = ctx ? ctx : &none;
+  p.kind = PK_MCCLEAN;
+  p.n = n;
+  p.L = L;
+  p.dtype = dtype;
+  p.ckpt_req = ckpt_every;
+  CHK(mcclean_geometry(&p));
+  *bytes = n <= kFusedMaxN ? 0 : p.ws_bytes;  // whole-circuit runs need no workspace
+  if (ckpt_chosen) *ckpt_chosen = p.history ? 0 : p.ckpt;
   return QG_OK;
 }
 
@@ -1541,6 +1712,7 @@ static int obs_tables(const ObsSplit& obs, FusedDescH& d, Arena& ar) {
 static int mcclean_run(qg_plan* p, const int32_t* axes, const double* angles, const qg_term* terms, int nterms,
                        const double* ini_state, double* E_out, double* grad_out, bool grad) {
   if (!p || p->kind != PK_MCCLEAN) return fail(QG_ERR_INVALID, "not a McClean plan");
+  if (p->mc_batch) return fail(QG_ERR_INVALID, "a batch plan runs through qg_mcclean_batch_grad_run");
   if (!axes || !angles || !E_out || (grad && !grad_out)) return fail(QG_ERR_INVALID, "NULL argument");
   const int n = p->n, L = p->L;
   for (int k = 0; k < L * n; ++k) {
@@ -1572,7 +1744,14 @@ static int mcclean_run(qg_plan* p, const int32_t* axes, const double* angles, co
       return obs_tables(obs, d, ar);
     };
     const size_t bytes = (size_t)L * n * 20 + (obs.diag.size() + obs.xy.size() + 2) * sizeof(DTerm);
-    if (!p->d_fstate) HIPCHK(hipMalloc(&p->d_fstate, ((size_t)16) << n));
+    if (p->cap_fstate < ((size_t)16 << n)) {
+      HIPCHK(hipStreamSynchronize(c->stream));
+      (void)hipFree(p->d_fstate);
+      p->d_fstate = nullptr;
+      p->cap_fstate = 0;
+      HIPCHK(hipMalloc(&p->d_fstate, ((size_t)16) << n));
+      p->cap_fstate = (size_t)16 << n;
+    }
     CHK(run_fused(p, 1, bytes, 1 + L * n, grad, p->d_fstate, fill, res.data()));
     *E_out = res[0];
     if (grad) std::memcpy(grad_out, res.data() + 1, sizeof(double) * L * n);
@@ -1657,6 +1836,251 @@ extern "C" int qg_mcclean_grad_run(qg_plan* p, const int32_t* axes, const double
 extern "C" int qg_mcclean_expec_val(qg_plan* p, const int32_t* axes, const double* angles, const qg_term* terms,
                                     int nterms, const double* ini_state, double* E_out) {
   return mcclean_run(p, axes, angles, terms, nterms, ini_state, E_out, nullptr, false);
+}
+
+// ---------------------------------------------------------------------------
+// Batched McClean: B circuits of one shape (their own axes and angles, one
+// observable), every pass one launch over the circuits of a chunk
+// ---------------------------------------------------------------------------
+// The tile shape, memory mode (the per-pass history) and per-circuit grid are
+// those of a single plan of the shape, so each circuit's result is bitwise
+// the one a single plan gives.
+static int mcclean_batch_layout(qg_plan* p) {
+  const int n = p->n, L = p->L;
+  p->kmax = default_kmax(n, p->dtype, p->cmin);
+  p->geo = layer_geometry(n, p->kmax, p->cmin);
+  const int P = (int)p->geo.size();
+  p->S = (size_t)(1ull << n) * (p->dtype == QG_C64 ? 8 : 16);
+  p->lds64 = p->dtype == QG_C64 && n >= 4 && P * L > 200;
+  p->history = true;
+  size_t budget = 0;
+  CHK(plan_budget(p, &budget));
+  const size_t per = (size_t)(P * L + 2) * p->S;  // history of every pass + two lambda buffers
+  if (n > kFusedMaxN) {
+    if (per > budget) return fail(QG_ERR_NOMEM, "one circuit's per-pass history does not fit in device memory");
+    p->chunk = (int)std::min<size_t>((size_t)p->mc_batch, budget / per);
+  } else {
+    p->chunk = p->mc_batch;  // one workgroup per circuit, no workspace
+  }
+  p->off_hist = 0;
+  p->off_lam = (size_t)p->chunk * P * L * p->S;
+  p->ws_bytes = n > kFusedMaxN ? (size_t)p->chunk * per : 0;
+  p->gmax = grid_max(p);
+  const int ch = n > kFusedMaxN ? p->chunk : 1;
+  p->nrows = ch * (P * L + 1) * (kMaxK + 1);
+  CHK(ensure_small(p, ch * 2 * P * L, p->nrows, ch * (1 + L * n)));
+  uint32_t* hm = (uint32_t*)p->h_tabs;
+  double* hi = (double*)(p->h_tabs + ((sizeof(uint32_t) * std::max(n, 1) + 15) & ~(size_t)15));
+  ladder_map(n, 1, hm);
+  ry_init_table(n, hi);
+  HIPCHK(hipMemcpyAsync(p->d_map, hm, sizeof(uint32_t) * n, hipMemcpyHostToDevice, p->ctx->stream));
+  HIPCHK(hipMemcpyAsync(p->d_init, hi, sizeof(double) * (n + 1), hipMemcpyHostToDevice, p->ctx->stream));
+  HIPCHK(hipStreamSynchronize(p->ctx->stream));
+  return QG_OK;
+}
+
+extern "C" int qg_mcclean_batch_plan(qg_ctx* ctx, int n, int L, int dtype, int entangler, int batch,
+                                     qg_plan** out) {
+  if (!ctx || !out) return fail(QG_ERR_INVALID, "NULL argument");
+  if (n < 1 || n > kMaxQubits) return fail(QG_ERR_INVALID, "qubit number must be in [1, 32]");
+  if (L < 1) return fail(QG_ERR_INVALID, "layer number must be >= 1");
+  if (dtype != QG_C128 && dtype != QG_C64) return fail(QG_ERR_INVALID, "dtype must be QG_C128 or QG_C64");
+  if (entangler != QG_ENT_CNOT && entangler != QG_ENT_CZ) return fail(QG_ERR_INVALID, "unknown entangler");
+  if (batch < 1) return fail(QG_ERR_INVALID, "batch must be >= 1");
+  HIPCHK(hipSetDevice(ctx->device));
+  qg_plan* p = new qg_plan();
+  p->ctx = ctx;
+  p->kind = PK_MCCLEAN;
+  p->n = n;
+  p->L = L;
+  p->dtype = dtype;
+  p->ent = entangler;
+  p->ckpt_req = QG_HISTORY;
+  p->mc_batch = batch;
+  const int r = mcclean_batch_layout(p);
+  if (r != QG_OK) {
+    qg_plan_destroy(p);
+    return r;
+  }
+  *out = p;
+  return QG_OK;
+}
+
+extern "C" int qg_mcclean_batch_grad_run(qg_plan* p, int B, const int32_t* axes, const double* angles,
+                                         const qg_term* terms, int nterms, double* E_out, double* grad_out) {
+  if (!p || p->kind != PK_MCCLEAN || !p->mc_batch) return fail(QG_ERR_INVALID, "not a McClean batch plan");
+  if (!axes || !angles || !E_out || !grad_out) return fail(QG_ERR_INVALID, "NULL argument");
+  if (B < 1 || B > p->mc_batch) return fail(QG_ERR_INVALID, "B must be in [1, the plan's batch]");
+  const int n = p->n, L = p->L, P = (int)p->geo.size(), per_out = 1 + L * n;
+  const size_t np = (size_t)L * n;
+  for (size_t k = 0; k < (size_t)B * np; ++k) {
+    if (axes[k] < 0 || axes[k] > 2) return fail(QG_ERR_INVALID, "Invalid axis " + std::to_string(axes[k]));
+    if (!std::isfinite(angles[k])) return fail(QG_ERR_INVALID, "non-finite angle");
+  }
+  ObsSplit obs;
+  CHK(split_observable(n, terms, nterms, obs));
+  if (!obs.xy.empty()) return fail(QG_ERR_UNSUPPORTED, "batched McClean runs take z / zz observables");
+  if ((int)obs.diag.size() > p->max_terms) return fail(QG_ERR_INVALID, "too many observable terms");
+  qg_ctx* c = p->ctx;
+  HIPCHK(hipSetDevice(c->device));
+  if (n <= kFusedMaxN) {
+    // the whole-circuit kernel, one workgroup per circuit, one launch
+    std::vector<double> res((size_t)B * per_out);
+    auto fill = [&](int b, FusedDescH& d, Arena& ar) -> int {
+      d.kind = 0;
+      int32_t* dt = nullptr;
+      int32_t* ht = ar.take<int32_t>(np, &dt);
+      double* dg = nullptr;
+      double* hg = ar.take<double>(2 * np, &dg);
+      const int32_t* ax = axes + (size_t)b * np;
+      const double* an = angles + (size_t)b * np;
+      for (size_t k = 0; k < np; ++k) {
+        ht[k] = ax[k] == 0 ? G_X : ax[k] == 1 ? G_Y : G_Z;
+        hg[2 * k] = std::cos(0.5 * an[k]);
+        hg[2 * k + 1] = std::sin(0.5 * an[k]);
+      }
+      d.gtype = dt;
+      d.gcs = dg;
+      d.psi_out = nullptr;
+      return obs_tables(obs, d, ar);
+    };
+    const size_t bytes = np * 20 + (obs.diag.size() + 2) * sizeof(DTerm);
+    CHK(run_fused(p, B, bytes, per_out, true, nullptr, fill, res.data()));
+    for (int b = 0; b < B; ++b) {
+      E_out[b] = res[(size_t)b * per_out];
+      std::memcpy(grad_out + (size_t)b * np, res.data() + (size_t)b * per_out + 1, sizeof(double) * np);
+    }
+    p->last_state_ptr = nullptr;
+    return QG_OK;
+  }
+  CHK(ensure_workspace(p));
+  if (!obs.diag.empty())
+    HIPCHK(hipMemcpyAsync(p->d_terms, obs.diag.data(), sizeof(DTerm) * obs.diag.size(), hipMemcpyHostToDevice,
+                          c->stream));
+  p->h_terms = obs.diag;
+  const uint32_t ent_fwd = p->ent == QG_ENT_CNOT ? F_GATHER : F_CZ_PRE;
+  const uint32_t ent_rev = p->ent == QG_ENT_CNOT ? F_SCATTER : F_CZ_POST;
+  float total_ms = 0.f;
+  for (int c0 = 0; c0 < B; c0 += p->chunk) {
+    const int nb = std::min(p->chunk, B - c0);
+    auto hist = [&](int j, int i, int g) { return wsp(p, p->off_hist + (((size_t)j * L + i) * P + g) * p->S); };
+    auto lam = [&](int j, int k) { return wsp(p, p->off_lam + ((size_t)j * 2 + k) * p->S); };
+    p->launches.clear();
+    int nd = 0, row = 0;
+    std::vector<std::vector<int>> out_rows((size_t)nb * per_out);
+    std::vector<int> cnt(p->nrows, 0);
+    // one launch per pass over the chunk's circuits, at a single plan's grid
+    auto begin = [&](const Geo& G0, bool two, int role, int* grid) -> Geo {
+      bool tf = false;
+      const Geo G = with_role(G0, p->dtype, role, &tf);
+      Launch l;
+      l.desc = nd;
+      l.batch = nb;
+      l.R = G.R;
+      l.K = G.K;
+      l.post = tf;
+      l.threads = G.threads;
+      l.grid = grid_for_pass(p, G, two, tf, 0);
+      l.phase = 0;
+      l.two = two;
+      l.kind = 0;
+      p->launches.push_back(l);
+      *grid = l.grid;
+      return G;
+    };
+    auto base = [&](const Geo& G) -> PassDesc& {
+      PassDesc& d = p->h_desc[nd++];
+      desc_geo(d, n, G);
+      d.grid = p->gmax;
+      d.map = p->d_map;
+      d.init_tab = p->d_init;
+      d.dterms = p->d_terms;
+      d.ndterms = (int)obs.diag.size();
+      d.part = p->d_part;
+      return d;
+    };
+    auto gates = [&](PassDesc& d, const Geo& G, int j, int i, double sign, int rb) {
+      const int32_t* ax = axes + (size_t)(c0 + j) * np;
+      const double* an = angles + (size_t)(c0 + j) * np;
+      for (int lb = G.own_lo; lb < G.own_hi; ++lb) {
+        const int b = (lb < G.c) ? lb : G.p + (lb - G.c);
+        const int q = n - 1 - b;
+        set_gate(d, lb, ax[i * n + q], an[i * n + q], sign);
+        if (rb >= 0) {
+          d.own[lb] = 1;
+          out_rows[(size_t)j * per_out + 1 + i * n + q].push_back(rb + lb);
+        }
+      }
+    };
+    // forward sweep; lambda = D psi and E ride on the last pass
+    for (int i = 0; i < L; ++i) {
+      for (int g = 0; g < P; ++g) {
+        const bool last = (i == L - 1) && (g == P - 1);
+        int gr = 0;
+        const Geo G = begin(p->geo[g], last, last ? ROLE_TWO_FP64 : ROLE_FWD, &gr);
+        for (int j = 0; j < nb; ++j) {
+          PassDesc& d = base(G);
+          uint32_t f = F_ROT_A | F_STORE_A;
+          if (g == 0) f |= ent_fwd | (i == 0 ? F_INIT_TAB : F_LOAD_A);
+          else f |= F_LOAD_A;
+          d.a_in = (i == 0 && g == 0) ? nullptr : (g == 0 ? hist(j, i - 1, P - 1) : hist(j, i, g - 1));
+          d.a_out = hist(j, i, g);
+          gates(d, p->geo[g], j, i, 1.0, -1);
+          if (last) {
+            f |= F_OBS_DIAG | F_STORE_B;
+            d.b_out = lam(j, 0);
+            d.row_base = row;
+            out_rows[(size_t)j * per_out].push_back(row + G.K);
+            for (int k = 0; k <= kMaxK; ++k) cnt[row + k] = gr;
+            row += kMaxK + 1;
+          }
+          d.flags = f;
+        }
+      }
+    }
+    // adjoint sweep (reduce-first history passes)
+    int lc = 0;
+    for (int i = L - 1; i >= 0; --i) {
+      for (int g = P - 1; g >= 0; --g) {
+        int gr = 0;
+        const Geo G = begin(p->geo[g], true, ROLE_REV_HIST, &gr);
+        const bool swap = g == 0 && p->ent == QG_ENT_CNOT;
+        for (int j = 0; j < nb; ++j) {
+          PassDesc& d = base(G);
+          uint32_t f = F_LOAD_A | F_ROT_A | F_LOAD_B | F_ROT_B | F_STORE_B | F_REDUCE;
+          if (g == 0) f |= ent_rev;
+          d.a_in = hist(j, i, g);
+          d.b_in = lam(j, lc);
+          d.b_out = lam(j, swap ? lc ^ 1 : lc);
+          d.flags = f;
+          d.row_base = row;
+          gates(d, p->geo[g], j, i, -1.0, row);
+          for (int k = 0; k <= kMaxK; ++k) cnt[row + k] = gr;
+          row += kMaxK + 1;
+        }
+        if (swap) lc ^= 1;
+      }
+    }
+    std::vector<double> scale(out_rows.size(), 1.0);
+    CHK(run_launches(p, nd));
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    CHK(collect_profile(p));
+    CHK(upload_finalize(p, out_rows, cnt, scale));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    total_ms += ms;
+    HIPCHK(hipMemcpyAsync(p->h_out, p->d_out, sizeof(double) * (size_t)nb * per_out, hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int j = 0; j < nb; ++j) {
+      const double* o = p->h_out + (size_t)j * per_out;
+      E_out[c0 + j] = o[0];
+      std::memcpy(grad_out + (size_t)(c0 + j) * np, o + 1, sizeof(double) * np);
+    }
+  }
+  p->last_ms = total_ms;
+  p->last_state_ptr = nullptr;
+  return QG_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -1896,6 +2320,47 @@ extern "C" int qg_mcclean_shift_sample(qg_plan* p, const int32_t* axes, const do
 // ---------------------------------------------------------------------------
 // QAOA (qradient/circuit_logic/qaoa.py:40-70), batched over circuits
 // ---------------------------------------------------------------------------
+// Geometry, chunk and tables of a QAOA plan (tile shape from kmax / cmin,
+// chunk from chunk_req and device memory).
+static int qaoa_layout(qg_plan* p) {
+  const int n = p->n, L = p->L;
+  if (p->kmax_auto) p->kmax = default_kmax(n, p->dtype, p->cmin);
+  p->geo = layer_geometry(n, p->kmax, p->cmin);
+  const int P = (int)p->geo.size();
+  p->S = (size_t)(1ull << n) * (p->dtype == QG_C64 ? 8 : 16);
+  const size_t per_circuit = (size_t)(P * L + 2) * p->S;
+  size_t budget = 0;
+  CHK(plan_budget(p, &budget));
+  if (per_circuit > budget)
+    return fail(QG_ERR_UNSUPPORTED, "one QAOA circuit's state history does not fit in device memory");
+  const int fit = (int)std::min<size_t>((size_t)p->batch, budget / per_circuit);
+  p->chunk = p->chunk_req > 0 ? std::min(p->chunk_req, fit) : fit;
+  p->off_hist = 0;
+  p->off_lam = (size_t)p->chunk * P * L * p->S;
+  p->ws_bytes = p->off_lam + (size_t)p->chunk * 2 * p->S;
+  p->gmax = grid_max(p);
+  const int ndesc = p->chunk * 2 * P * L;
+  p->nrows = p->chunk * (P * L + 1) * (kMaxK + 1);
+  CHK(ensure_small(p, ndesc, p->nrows, p->chunk * (1 + 2 * L)));
+  // each circuit of a chunk has its own classical Hamiltonian
+  const int mt = n + n * (n - 1) / 2;
+  const size_t need = (size_t)mt * p->chunk;
+  if (need > p->qaoa_terms_cap) {
+    HIPCHK(hipStreamSynchronize(p->ctx->stream));
+    (void)hipFree(p->d_terms);
+    p->d_terms = nullptr;
+    p->qaoa_terms_cap = 0;
+    HIPCHK(hipMalloc(&p->d_terms, sizeof(DTerm) * need));
+    p->qaoa_terms_cap = need;
+  }
+  p->max_terms = mt;
+  double* hi = (double*)(p->h_tabs + ((sizeof(uint32_t) * std::max(n, 1) + 15) & ~(size_t)15));
+  for (int k = 0; k <= n; ++k) hi[k] = std::pow(2.0, -0.5 * n);
+  HIPCHK(hipMemcpyAsync(p->d_init, hi, sizeof(double) * (n + 1), hipMemcpyHostToDevice, p->ctx->stream));
+  HIPCHK(hipStreamSynchronize(p->ctx->stream));
+  return QG_OK;
+}
+
 extern "C" int qg_qaoa_plan(qg_ctx* ctx, int n, int p_layers, int dtype, int batch, qg_plan** out) {
   if (!ctx || !out) return fail(QG_ERR_INVALID, "NULL argument");
   if (n < 1 || n > kMaxQubits) return fail(QG_ERR_INVALID, "qubit number must be in [1, 32]");
@@ -1910,44 +2375,43 @@ extern "C" int qg_qaoa_plan(qg_ctx* ctx, int n, int p_layers, int dtype, int bat
   p->L = p_layers;
   p->dtype = dtype;
   p->batch = batch;
-  p->kmax = default_kmax(n, dtype, p->cmin);
-  p->geo = layer_geometry(n, p->kmax, p->cmin);
-  const int P = (int)p->geo.size();
-  p->S = (size_t)(1ull << n) * (dtype == QG_C64 ? 8 : 16);
-  const size_t per_circuit = (size_t)(P * p_layers + 2) * p->S;
-  size_t free_b = 0, total_b = 0;
-  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
-    delete p;
-    return fail(QG_ERR_HIP, "hipMemGetInfo failed");
-  }
-  const size_t budget = (size_t)(0.80 * (double)free_b);
-  if (per_circuit > budget) {
-    delete p;
-    return fail(QG_ERR_UNSUPPORTED, "one QAOA circuit's state history does not fit in device memory");
-  }
-  p->chunk = (int)std::min<size_t>((size_t)batch, budget / per_circuit);
-  p->off_hist = 0;
-  p->off_lam = (size_t)p->chunk * P * p_layers * p->S;
-  p->ws_bytes = p->off_lam + (size_t)p->chunk * 2 * p->S;
-  p->gmax = grid_max(p);
-  const int ndesc = p->chunk * 2 * P * p_layers;
-  p->nrows = p->chunk * (P * p_layers + 1) * (kMaxK + 1);
-  int r = alloc_small(p, ndesc, p->nrows, p->chunk * (1 + 2 * p_layers));
+  p->wg_per_cu = kBatchWgPerCu;
+  const int r = qaoa_layout(p);
   if (r != QG_OK) {
     qg_plan_destroy(p);
     return r;
   }
-  (void)hipFree(p->d_terms);
-  p->d_terms = nullptr;
-  p->max_terms = n + n * (n - 1) / 2;
-  hipError_t e = hipMalloc(&p->d_terms, sizeof(DTerm) * (size_t)p->max_terms * p->chunk);
-  std::vector<double> init(n + 1, std::pow(2.0, -0.5 * n));
-  if (e == hipSuccess) e = hipMemcpy(p->d_init, init.data(), sizeof(double) * (n + 1), hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    qg_plan_destroy(p);
-    return fail(QG_ERR_HIP, std::string("qaoa plan tables: ") + hipGetErrorString(e));
-  }
   *out = p;
+  return QG_OK;
+}
+
+extern "C" int qg_qaoa_tune(qg_plan* p, int kmax, int cmin, int chunk, int wg_per_cu) {
+  if (!p || p->kind != PK_QAOA) return fail(QG_ERR_INVALID, "not a QAOA plan");
+  if (kmax != 0 && (kmax < 4 || kmax > 13)) return fail(QG_ERR_INVALID, "kmax must be 0 (default) or in [4, 13]");
+  if (cmin < 0 || (kmax != 0 && cmin >= kmax)) return fail(QG_ERR_INVALID, "cmin must be in [0, kmax)");
+  if (chunk < 0) return fail(QG_ERR_INVALID, "chunk must be >= 0 (0: as many as fit)");
+  if (wg_per_cu < 1 || wg_per_cu > 64) return fail(QG_ERR_INVALID, "wg_per_cu must be in [1, 64]");
+  HIPCHK(hipSetDevice(p->ctx->device));
+  HIPCHK(hipStreamSynchronize(p->ctx->stream));
+  drop_graph(p);
+  p->kmax_auto = kmax == 0;
+  if (kmax) p->kmax = kmax;
+  p->cmin = cmin;
+  p->chunk_req = chunk;
+  p->wg_per_cu = wg_per_cu;
+  if (p->ws && p->ws_owned) {
+    (void)hipFree(p->ws);
+    p->ws = nullptr;
+    p->ws_cap = 0;
+    p->ws_owned = false;
+  }
+  const size_t old_ws = p->ws_cap;
+  CHK(qaoa_layout(p));
+  if (p->ws && p->ws_bytes > old_ws) {
+    p->ws = nullptr;  // a supplied workspace that is too small is detached
+    p->ws_cap = 0;
+  }
+  p->last_state_ptr = nullptr;
   return QG_OK;
 }
 
@@ -2002,7 +2466,14 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
     for (int b = 0; b < B; ++b) mx = std::max(mx, obs[b].diag.size());
     const size_t bytes = (size_t)L * 16 + (mx + 2) * sizeof(DTerm);
     // a single circuit keeps its final state (qg_plan_get_state, qg_plan_term_prob)
-    if (B == 1 && !p->d_fstate) HIPCHK(hipMalloc(&p->d_fstate, ((size_t)16) << n));
+    if (B == 1 && p->cap_fstate < ((size_t)16 << n)) {
+      HIPCHK(hipStreamSynchronize(c->stream));
+      (void)hipFree(p->d_fstate);
+      p->d_fstate = nullptr;
+      p->cap_fstate = 0;
+      HIPCHK(hipMalloc(&p->d_fstate, ((size_t)16) << n));
+      p->cap_fstate = (size_t)16 << n;
+    }
     CHK(run_fused(p, B, bytes, per, grad, B == 1 ? p->d_fstate : nullptr, fill, res.data()));
     for (int b = 0; b < B; ++b) {
       E_out[b] = res[(size_t)b * per];
@@ -2016,9 +2487,14 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
   CHK(ensure_workspace(p));
   const int chunk = p->chunk;
   const int lut_len = 2 * Wi + 1;
-  if (ini && !p->d_ini_states) {
+  if (ini && p->ini_cap < (size_t)chunk * p->S) {
     // one initial state per circuit of a chunk, in the storage type
+    HIPCHK(hipStreamSynchronize(c->stream));
+    (void)hipFree(p->d_ini_states);
+    p->d_ini_states = nullptr;
+    p->ini_cap = 0;
     HIPCHK(hipMalloc(&p->d_ini_states, (size_t)chunk * p->S));
+    p->ini_cap = (size_t)chunk * p->S;
   }
   if (phase == 1) {
     const int need = chunk * 2 * L * lut_len * 2;
@@ -2122,9 +2598,9 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
       l.grid = grid_for_pass(p, G, two, tf, phase);
       // a batched launch fills the device across circuits: fewer workgroups
       // per circuit, each running more tiles (amortises its prologue); up to
-      // kBatchWgPerCu per CU in all, beyond the resident count, so that the
+      // wg_per_cu per CU in all, beyond the resident count, so that the
       // circuits' tiles spread evenly over the CUs
-      while (l.grid > 1 && (uint64_t)l.grid * nb > (uint64_t)kBatchWgPerCu * p->ctx->num_cu) l.grid >>= 1;
+      while (l.grid > 1 && (uint64_t)l.grid * nb > (uint64_t)p->wg_per_cu * p->ctx->num_cu) l.grid >>= 1;
       l.phase = phase;
       l.two = two;
       l.kind = 0;
@@ -2579,20 +3055,38 @@ extern "C" int qg_state_scale(qg_state* s, double re, double im) {
 }
 
 // Lays a McClean plan out again after a change of its shape parameters.
-static int relayout(qg_plan* p) {
+// Lays a McClean plan out again (new shape or tile options).  An owned
+// workspace too small for the new layout is freed (the next run allocates);
+// a supplied one is detached when detach_small is set, else it is an error.
+static int relayout(qg_plan* p, bool detach_small = false) {
   HIPCHK(hipSetDevice(p->ctx->device));
   HIPCHK(hipStreamSynchronize(p->ctx->stream));
-  const size_t old_ws = p->ws_bytes;
-  free_small(p);
+  drop_graph(p);
   CHK(mcclean_layout(p));
-  if (p->ws && p->ws_bytes > old_ws) {
-    if (!p->ws_owned) return fail(QG_ERR_INVALID, "the new layout needs a larger workspace than the one supplied");
-    (void)hipFree(p->ws);
+  if (p->ws && p->ws_bytes > p->ws_cap) {
+    if (p->ws_owned) {
+      (void)hipFree(p->ws);
+    } else if (!detach_small) {
+      return fail(QG_ERR_INVALID, "the new layout needs a larger workspace than the one supplied");
+    }
     p->ws = nullptr;
+    p->ws_cap = 0;
     p->ws_owned = false;
   }
   p->last_state_ptr = nullptr;
   return QG_OK;
+}
+
+extern "C" int qg_plan_reshape(qg_plan* p, int n, int L, int ckpt_every) {
+  if (!p) return fail(QG_ERR_INVALID, "NULL plan");
+  if (p->kind != PK_MCCLEAN || p->mc_batch) return fail(QG_ERR_UNSUPPORTED, "qg_plan_reshape applies to McClean plans");
+  if (n < 1 || n > kMaxQubits) return fail(QG_ERR_INVALID, "qubit number must be in [1, 32]");
+  if (L < 1) return fail(QG_ERR_INVALID, "layer number must be >= 1");
+  if (ckpt_every < QG_HISTORY) return fail(QG_ERR_INVALID, "ckpt_every must be >= 0 or QG_HISTORY");
+  p->n = n;
+  p->L = L;
+  p->ckpt_req = ckpt_every;
+  return relayout(p, true);
 }
 
 extern "C" int qg_plan_set_graph(qg_plan* p, int on) {
@@ -2607,6 +3101,7 @@ extern "C" int qg_plan_set_graph(qg_plan* p, int on) {
 extern "C" int qg_plan_set_lds64(qg_plan* p, int mode) {
   if (!p) return fail(QG_ERR_INVALID, "NULL plan");
   if (mode < -1 || mode > 1) return fail(QG_ERR_INVALID, "mode must be -1, 0 or 1");
+  if (p->mc_batch) return fail(QG_ERR_UNSUPPORTED, "batch plans keep the default tile shape");
   if (p->kind != PK_MCCLEAN) return fail(QG_ERR_UNSUPPORTED, "FP64 tiles apply to McClean plans");
   p->lds64_req = mode;
   return relayout(p);
@@ -2624,6 +3119,7 @@ extern "C" int qg_plan_tune(qg_plan* p, int kmax, int cmin, int r_fwd, int r_rev
   if (kmax > 13 && p->dtype != QG_C64) return fail(QG_ERR_INVALID, "kmax 14 needs a QG_C64 plan");
   if (cmin < 0 || cmin >= kmax) return fail(QG_ERR_INVALID, "cmin must be in [0, kmax)");
   if (r_fwd != 4 || r_rev != 4) return fail(QG_ERR_INVALID, "r_fwd, r_rev must be 4 (16 amplitudes per thread)");
+  if (p->mc_batch) return fail(QG_ERR_UNSUPPORTED, "batch plans keep the default tile shape");
   if (p->kind != PK_MCCLEAN) return fail(QG_ERR_UNSUPPORTED, "tuning applies to McClean plans");
   p->kmax = kmax;
   p->kmax_auto = false;

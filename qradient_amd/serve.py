@@ -1,23 +1,19 @@
-"""Continuous-batching grad_run server.
+"""Continuous-batching grad_run server (Python face of the library's server).
 
-Requests (McClean circuits of any qubit count and depth) are admitted while
-device memory and worker slots allow and run concurrently: each slot owns a
-library context with its own HIP stream, so kernels of different requests
-overlap on the GPU (ctypes releases the GIL during the calls).
+The scheduling runs in the library (qg_server_create / qg_submit / qg_poll /
+qg_wait_any, declared in include/qradient_amd.h), so callers in any language
+get the same server: one device arena allocated at creation, worker slots
+with their own HIP streams and one re-laid-out plan each (no device
+allocation on a shape change once the plans have grown), admission by the
+workspace footprint the library's layout rule gives, bounded backfill past a
+head-of-queue request that does not fit, and a memory mode fixed by the
+request's shape, so a result is bitwise the same alone and in any mix.
 
-Admission is by HBM footprint: the statevector workspace of the request's
-plan.  A request waits until its footprint fits the budget left by the other
-slots; queued requests that fit may overtake a larger head-of-queue request at
-most ``backfill_limit`` times, after which capacity is held for the head (no
-starvation).  Each slot keeps the plan (device workspace) of its last shape
-resident, so back-to-back requests of one shape reuse their states; the
-memory a slot holds counts against the budget until it switches shape.
-
-The memory mode of a request depends only on its shape (full history below
-``history_max_bytes``, otherwise checkpoints every ``ckpt_every`` layers), so
-its result is bitwise the same whether it runs alone or in a mix.
+``GradServer.submit`` returns a Future resolving to (E, grad[L, n]); a poller
+thread collects finished requests from the library.  ``simulate=True`` runs
+the library's scheduler without a device (requests return E = n, grad = L):
+the admission logic is testable on hosts without a GPU.
 """
-import collections
 import ctypes
 import threading
 import time
@@ -38,204 +34,135 @@ class Request:
         self.angles = np.ascontiguousarray(angles, dtype=np.float64).reshape(self.L, self.n)
         self.observable = observable if isinstance(observable, Observable) else Observable(observable)
         self.tag = tag
-        self.arrival = self.start = self.finish = None
-        self.footprint = None
-        self.mode = None
+        self.arrival = self.finish = None
+        self.info = None      # the library's qg_request_info, as a dict, once finished
         self.future = Future()
 
 
-def passes_per_layer(n, kmax=13, cmin=3):
-    return 1 if n <= kmax else 1 + -(-(n - kmax) // (kmax - cmin))
-
-
-class LibraryExecutor:
-    """Runs requests through libqradient_amd, one context (stream) per slot."""
-
-    def __init__(self, slots, device=0, dtype='complex64'):
-        self.dtype = dtype
-        self.ctx = [_native.Context(device) for _ in range(slots)]
-        self.plan = [None] * slots     # (shape key, handle)
-
-    def state_bytes(self, n):
-        return (2 ** n) * (8 if self.dtype == 'complex64' else 16)
-
-    def release(self, si):
-        if self.plan[si] is not None:
-            _native.load().qg_plan_destroy(self.plan[si][1])
-            self.plan[si] = None
-
-    def run(self, si, req):
-        key = (req.n, req.L, req.mode)
-        if self.plan[si] is None or self.plan[si][0] != key:
-            self.release(si)
-            h = ctypes.c_void_p()
-            # mode 0 = the history, forced so the numerics depend on the shape only
-            _native.call('qg_mcclean_plan', self.ctx[si].handle, req.n, req.L, _native.DTYPES[self.dtype],
-                         _native.QG_ENT_CNOT, req.mode if req.mode > 0 else _native.QG_HISTORY, ctypes.byref(h))
-            self.plan[si] = (key, h)
-        arr, k = req.observable.c_terms()
-        E = ctypes.c_double()
-        grad = np.empty((req.L, req.n))
-        _native.call('qg_mcclean_grad_run', self.plan[si][1], _native.iptr(req.axes), _native.dptr(req.angles), arr,
-                     k, None, ctypes.byref(E), _native.dptr(grad))
-        return float(E.value), grad
-
-    def close(self):
-        for si in range(len(self.plan)):
-            self.release(si)
+def _info_dict(info):
+    return {name: getattr(info, name) for name, _ in _native.RequestInfo._fields_}
 
 
 class GradServer:
+    """``budget_bytes``: the device arena (default 80% of the free memory);
+    ``history_max_bytes`` / ``ckpt_every``: the shape-to-memory-mode rule;
+    ``backfill_limit``: how often a head-of-queue request may be overtaken."""
+
     def __init__(self, device=0, slots=4, budget_bytes=None, dtype='complex64', history_max_bytes=4 << 30,
-                 ckpt_every=8, backfill_limit=8, executor=None):
+                 ckpt_every=8, backfill_limit=8, simulate=False):
         if dtype not in _native.DTYPES:
             raise ValueError('dtype must be complex128 or complex64')
-        self.dtype = dtype
-        self.history_max_bytes = int(history_max_bytes)
-        self.ckpt_every = int(ckpt_every)
-        self.backfill_limit = int(backfill_limit)
-        self.nslots = int(slots)
-        self.executor = executor if executor is not None else LibraryExecutor(self.nslots, device, dtype)
         if budget_bytes is None:
-            import torch
-            free, _ = torch.cuda.mem_get_info(device)
-            budget_bytes = int(0.8 * free)
+            if simulate:
+                budget_bytes = 1 << 40
+            else:
+                import torch
+                free, _ = torch.cuda.mem_get_info(device)
+                budget_bytes = int(0.8 * free)
+        self.dtype = dtype
         self.budget = int(budget_bytes)
-        self._held = [0] * self.nslots          # bytes a slot's resident plan occupies
-        self._shape = [None] * self.nslots      # shape key of that plan
-        self._busy = [False] * self.nslots
-        self._queue = collections.deque()
-        self._cv = threading.Condition()
-        self._closed = False
-        self._bypassed = 0
-        self.completed = []
-        self.peak_bytes = 0
-        self._workers = []
-        self._dispatcher = threading.Thread(target=self._dispatch, daemon=True)
-        self._dispatcher.start()
+        self.nslots = int(slots)
+        h = ctypes.c_void_p()
+        _native.call('qg_server_create', int(device), self.nslots, _native.DTYPES[dtype], self.budget,
+                     int(history_max_bytes), int(ckpt_every), int(backfill_limit),
+                     _native.QG_SERVER_SIMULATE if simulate else 0, ctypes.byref(h))
+        self._h = h
+        self._lock = threading.Lock()
+        self._pending = {}        # ticket -> Request
+        self._stop = False
+        self.completed = []       # requests in completion order
+        self._poller = threading.Thread(target=self._poll_loop, daemon=True)
+        self._poller.start()
 
-    # ---- sizing ---------------------------------------------------------------
-    def _state_bytes(self, n):
-        return (2 ** n) * (8 if self.dtype == 'complex64' else 16)
+    # ---- sizing (the library's rule) ---------------------------------------------
+    def footprint(self, n, L):
+        b = ctypes.c_size_t()
+        k = ctypes.c_int32()
+        _native.call('qg_server_footprint', self._h, int(n), int(L), ctypes.byref(b), ctypes.byref(k))
+        return int(b.value)
 
     def mode_of(self, n, L):
-        """0 (full history) when it fits history_max_bytes, else the checkpoint period."""
-        P = passes_per_layer(n)
-        return 0 if (P * L + 6) * self._state_bytes(n) <= self.history_max_bytes else self.ckpt_every
+        """0 (per-pass history) or the checkpoint period of a request shape."""
+        b = ctypes.c_size_t()
+        k = ctypes.c_int32()
+        _native.call('qg_server_footprint', self._h, int(n), int(L), ctypes.byref(b), ctypes.byref(k))
+        return int(k.value)
 
-    def footprint(self, n, L):
-        """Workspace bytes of the plan for a request shape (the library's layout)."""
-        S = self._state_bytes(n)
-        mode = self.mode_of(n, L)
-        if mode == 0:
-            return (passes_per_layer(n) * L + 4) * S
-        nck = sum(1 for i in range(L) if (i + 1) % mode == 0 or i == L - 1)
-        return (2 + nck + 4) * S
-
-    # ---- public API -------------------------------------------------------------
+    # ---- requests ----------------------------------------------------------------
     def submit(self, req):
         """Queue a request; returns a Future resolving to (E, grad[L, n])."""
         req.arrival = time.perf_counter()
-        req.mode = self.mode_of(req.n, req.L)
-        req.footprint = self.footprint(req.n, req.L)
-        if req.footprint > self.budget:
-            req.future.set_exception(MemoryError('request needs {} bytes, the budget is {}'.format(
-                req.footprint, self.budget)))
-            return req.future
-        with self._cv:
-            if self._closed:
-                raise RuntimeError('server is closed')
-            self._queue.append(req)
-            self._cv.notify_all()
+        arr, k = req.observable.c_terms()
+        t = ctypes.c_int64()
+        lib = _native.load()
+        with self._lock:
+            status = lib.qg_submit(self._h, req.n, req.L, _native.iptr(req.axes), _native.dptr(req.angles), arr, k,
+                                   ctypes.byref(t))
+            if status == _native.QG_OK:
+                self._pending[t.value] = req
+        if status != _native.QG_OK:
+            try:
+                _native.check(status)
+            except Exception as e:  # noqa: BLE001 - reported through the future
+                req.future.set_exception(e)
         return req.future
 
+    def _poll_loop(self):
+        lib = _native.load()
+        t = ctypes.c_int64()
+        info = _native.RequestInfo()
+        while True:
+            status = lib.qg_wait_any(self._h, 100, ctypes.byref(t))
+            if status == _native.QG_ERR_PENDING:
+                with self._lock:
+                    if self._stop and not self._pending:
+                        return
+                continue
+            with self._lock:
+                req = self._pending.pop(t.value)
+            E = ctypes.c_double()
+            grad = np.empty((req.L, req.n))
+            status = lib.qg_poll(self._h, t.value, -1, ctypes.byref(E), _native.dptr(grad), ctypes.byref(info))
+            req.info = _info_dict(info)
+            req.finish = req.arrival + req.info['latency_ms'] * 1e-3
+            self.completed.append(req)
+            if status == _native.QG_OK:
+                req.future.set_result((float(E.value), grad))
+            else:
+                try:
+                    _native.check(status)
+                except Exception as e:  # noqa: BLE001 - reported through the future
+                    req.future.set_exception(e)
+
     def close(self, wait=True):
-        with self._cv:
-            self._closed = True
-            self._cv.notify_all()
         if wait:
-            self._dispatcher.join()
-            for t in list(self._workers):
-                t.join()
-        self.executor.close()
+            with self._lock:
+                futs = [r.future for r in self._pending.values()]
+            for f in futs:
+                f.exception()
+        with self._lock:
+            self._stop = True
+        self._poller.join()
+        h, self._h = self._h, None
+        if h:
+            _native.call('qg_server_destroy', h)
+
+    def counters(self):
+        c = _native.ServerCounters()
+        _native.call('qg_server_stats', self._h, ctypes.byref(c))
+        return {name: int(getattr(c, name)) for name, _ in _native.ServerCounters._fields_}
 
     def stats(self):
-        done = [r for r in self.completed if r.finish is not None]
+        done = [r for r in self.completed if r.info is not None]
         if not done:
             return {}
-        lat = np.array([r.finish - r.arrival for r in done])
+        lat = np.array([r.info['latency_ms'] for r in done])
         t0, t1 = min(r.arrival for r in done), max(r.finish for r in done)
-        return {'requests': int(lat.size), 'p50_ms': float(np.percentile(lat, 50) * 1e3),
-                'p99_ms': float(np.percentile(lat, 99) * 1e3), 'mean_ms': float(lat.mean() * 1e3),
+        c = self.counters() if self._h else {'arena_peak': None}
+        return {'requests': int(lat.size), 'p50_ms': float(np.percentile(lat, 50)),
+                'p99_ms': float(np.percentile(lat, 99)), 'mean_ms': float(lat.mean()),
                 'circuits_per_s': float(lat.size / (t1 - t0)) if t1 > t0 else float('nan'),
-                'peak_bytes': int(self.peak_bytes), 'budget_bytes': self.budget}
-
-    # ---- scheduling (lock held) ---------------------------------------------------
-    def _slot_for(self, req):
-        """A free slot for req whose admission keeps the held memory within budget, or None."""
-        key = (req.n, req.L, req.mode)
-        best = None
-        for si in range(self.nslots):
-            if self._busy[si]:
-                continue
-            held_others = sum(self._held) - self._held[si]
-            if held_others + req.footprint > self.budget:
-                continue
-            if self._shape[si] == key:
-                return si            # reuse the resident plan of this shape
-            if best is None or self._held[si] > self._held[best]:
-                best = si            # otherwise free the largest idle plan
-        return best
-
-    def _pick(self):
-        for i, r in enumerate(self._queue):
-            si = self._slot_for(r)
-            if si is not None:
-                self._bypassed = self._bypassed + 1 if i > 0 else 0
-                return i, si
-            if i == 0 and self._bypassed >= self.backfill_limit:
-                return None          # hold capacity for the head of the queue
-        return None
-
-    def _dispatch(self):
-        while True:
-            with self._cv:
-                while True:
-                    if self._closed and not self._queue:
-                        return
-                    pick = self._pick() if not all(self._busy) else None
-                    if pick is not None:
-                        break
-                    self._cv.wait()
-                i, si = pick
-                req = self._queue[i]
-                del self._queue[i]
-                self._busy[si] = True
-                self._shape[si] = (req.n, req.L, req.mode)
-                self._held[si] = req.footprint
-                self.peak_bytes = max(self.peak_bytes, sum(self._held))
-            t = threading.Thread(target=self._run, args=(si, req), daemon=True)
-            self._workers = [x for x in self._workers if x.is_alive()] + [t]
-            t.start()
-
-    def _run(self, si, req):
-        try:
-            req.start = time.perf_counter()
-            out = self.executor.run(si, req)
-            req.finish = time.perf_counter()
-            req.future.set_result(out)
-        except BaseException as e:
-            req.finish = time.perf_counter()
-            self.executor.release(si)
-            with self._cv:
-                self._held[si] = 0
-                self._shape[si] = None
-            req.future.set_exception(e)
-        finally:
-            with self._cv:
-                self._busy[si] = False
-                self.completed.append(req)
-                self._cv.notify_all()
+                'peak_bytes': c['arena_peak'], 'budget_bytes': self.budget}
 
 
 def poisson_trace(num, rate, seed, n_range=(12, 28), L_range=(4, 64)):

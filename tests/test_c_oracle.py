@@ -51,3 +51,28 @@ def test_qaoa_c_equals_numpy(n, p, seed):
     Eo, go = sv.qaoa_grad_run(n, obs, betas, gammas)
     assert abs(E - Eo) <= 1e-12 * max(1.0, abs(Eo))
     assert rel_err(g, go) <= 1e-12
+
+
+@pytest.mark.parametrize('n,L,seed', [(6, 5, 11), (12, 6, 12), (16, 4, 13)])
+def test_extended_precision_build_is_the_arbiter(n, L, seed):
+    """The x87 build of the same algorithm agrees with both FP64 formulations
+    to well within 1e-12 at small depth, and is closer to the matrix-free
+    adjoint (fewer roundings per gradient) than the FP64 history order is."""
+    rs = np.random.RandomState(seed)
+    axes, angles = sv.mcclean_random_params(L, n, rs)
+    obs = sv.zz_chain_observable(n, rs.standard_normal(n - 1))
+    Ex, gx = c_oracle.mcclean_grad_run(n, axes, angles, obs, extended=True)
+    E, g = c_oracle.mcclean_grad_run(n, axes, angles, obs)
+    Ea, ga = sv.mcclean_grad_run_adjoint(n, axes, angles, obs)
+    assert abs(Ex - E) <= 1e-14 and abs(Ex - Ea) <= 1e-14
+    assert rel_err(g, gx) <= 1e-13 and rel_err(ga, gx) <= 1e-13
+
+
+def test_extended_precision_qaoa():
+    n, p = 8, 3
+    rs = np.random.RandomState(14)
+    obs = sv.maxcut_observable(n, sv.random_regular3_edges(n, rs))
+    betas, gammas = np.pi * rs.rand(p), np.pi * rs.rand(p)
+    Ex, gx = c_oracle.qaoa_grad_run(n, obs, betas, gammas, extended=True)
+    Eo, go = sv.qaoa_grad_run(n, obs, betas, gammas)
+    assert abs(Ex - Eo) <= 1e-13 and rel_err(go, gx) <= 1e-13

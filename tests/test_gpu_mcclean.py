@@ -255,3 +255,42 @@ def test_states_of_4_gib_and_more(gpu, n, lds64):
     assert abs(E1 - E2) <= C64_TOL * max(1.0, abs(E1))
     assert rel_err(g2, g1) <= C64_TOL
     assert np.all(np.isfinite(g1)) and np.linalg.norm(g1) > 0
+
+
+def test_plan_mode_and_bits_do_not_depend_on_free_memory(gpu):
+    """The planner picks history or checkpoints from the circuit shape and the
+    device's total memory.  A 26-qubit, 64-layer complex64 circuit keeps its
+    per-pass history (3 passes x 64 layers + 4 states of 512 MiB, 98 GiB) on an
+    idle device; a plan made while an unrelated allocation leaves only that
+    much plus 10 GiB free must choose the same mode and give bitwise the same
+    result (a planner reading free memory would switch to checkpoints)."""
+    import ctypes
+    import torch
+    from qradient_amd import _native
+    from qradient_amd.circuit_logic import McClean
+    n, L = 26, 64
+    axes, angles, obs = _case(n, L, 41)
+    hist_bytes = (3 * L + 4) * (2 ** n) * 8
+
+    def run():
+        c = McClean(n, obs, L, axes=axes, angles=angles, dtype='complex64')
+        E, g = c.grad_run()
+        k = ctypes.c_int32()
+        _native.call('qg_plan_ckpt_every', c._get_plan(), ctypes.byref(k))
+        c._destroy_plan()
+        return E, g, k.value
+
+    E0, g0, k0 = run()
+    assert k0 == 0  # history
+    free, total = torch.cuda.mem_get_info(0)
+    hold_bytes = free - hist_bytes - (10 << 30)
+    hold = torch.empty(hold_bytes // 4, dtype=torch.float32, device='cuda')
+    try:
+        free_now, _ = torch.cuda.mem_get_info(0)
+        assert 0.85 * free_now < hist_bytes  # the old free-memory rule would checkpoint here
+        E1, g1, k1 = run()
+    finally:
+        del hold
+        torch.cuda.empty_cache()
+    assert k0 == k1
+    assert E0 == E1 and np.array_equal(g0, g1)

@@ -114,6 +114,24 @@ int qg_mcclean_workspace_bytes(qg_ctx* ctx, int n, int L, int dtype, int ckpt_ev
 int qg_mcclean_grad_run(qg_plan* plan, const int32_t* axes, const double* angles, const qg_term* terms,
                         int nterms, const double* ini_state, double* E_out, double* grad_out);
 
+/* Allocation-free McClean grad run.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It is the
+ * whole-circuit path of qg_mcclean_grad_run (n <= 13, no ini_state, one
+ * workgroup, FP64 throughout, so the result does not depend on a dtype) and
+ * returns bitwise the same E and grad[L*n] as a plan of that shape, without a
+ * plan: the tables and results live in one static device array per GPU of
+ * QG_NOALLOC_SCRATCH_BYTES, whose address the library takes at
+ * qg_ctx_create.  It calls no hipMalloc, hipFree or other allocating routine.
+ * Circuits whose tables exceed the array (about 28 bytes per parameter plus
+ * the observable) are rejected with QG_ERR_INVALID, as is n > 13.  Every call
+ * on a GPU shares that GPU's array, which keeps the last call's contents:
+ * calls that overlap on one GPU (from any context or thread) race. */
+#define QG_NOALLOC_SCRATCH_BYTES (256u * 1024u)
+int qg_mcclean_grad_run_noalloc(qg_ctx* ctx, int n, int L, int entangler, const int32_t* axes,
+                                const double* angles, const qg_term* terms, int nterms, double* E_out,
+                                double* grad_out);
+
 /* McClean.run_expec_val with exact_expec_val=True (mc_clean.py:27-45). */
 int qg_mcclean_expec_val(qg_plan* plan, const int32_t* axes, const double* angles, const qg_term* terms,
                          int nterms, const double* ini_state, double* E_out);

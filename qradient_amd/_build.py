@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, 'csrc')
 BUILD = os.path.join(HERE, '_build')
 LIB = os.path.join(HERE, 'libqradient_amd.so')
 SOURCES = ['qg_kernels_c64.hip', 'qg_kernels_c64x.hip', 'qg_kernels_c128.hip', 'qg_finalize.hip', 'qg_fused.hip', 'qg_simple.hip',
-           'qg_stream.hip', 'qg_api.cpp', 'qg_server.cpp']
+           'qg_noalloc.hip', 'qg_stream.hip', 'qg_api.cpp', 'qg_server.cpp']
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC',
          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',

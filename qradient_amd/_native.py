@@ -60,6 +60,7 @@ SIGNATURES = {
     'qg_mcclean_grad_run': [_vp, _ip, _dp, _tp, _i, _dp, _dp, _dp],
     'qg_mcclean_workspace_bytes': [_vp, _i, _i, _i, _i, ctypes.POINTER(ctypes.c_size_t), _ip],
     'qg_mcclean_expec_val': [_vp, _ip, _dp, _tp, _i, _dp, _dp],
+    'qg_mcclean_grad_run_noalloc': [_vp, _i, _i, _i, _ip, _dp, _tp, _i, _dp, _dp],
     'qg_mcclean_shift_term_prob': [_vp, _ip, _dp, _dp, _i, _ip, _dp, _tp, _i, _dp],
     'qg_mcclean_shift_sample': [_vp, _ip, _dp, _dp, _i, _ip, _dp, _ip, _i, _dp, _ip],
     'qg_state_sample_basis': [_vp, _ip, _i, _dp, _ip],

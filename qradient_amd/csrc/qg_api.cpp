@@ -40,7 +40,9 @@ constexpr int kDotBlocks = 512;
 hipError_t launch_fused(const void* descs, int batch, int n, hipStream_t st);
 hipError_t set_fused_attributes();
 size_t fused_desc_size();
+hipError_t noalloc_scratch_init(int device);
 hipError_t launch_stream_copy(void* dst, const void* src, size_t bytes, bool nontemporal, int grid, hipStream_t st);
+void* noalloc_scratch(int device);
 
 // Mirror of the device-side FusedDesc in qg_fused.hip (same layout).
 struct FusedDescH {
@@ -107,6 +109,9 @@ struct qg_ctx {
   size_t max_lds = 65536;
   size_t total_mem = 0;  // device memory, queried once (plan layouts are sized against it)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // pinned host mirror of the device scratch of qg_mcclean_grad_run_noalloc
+  // (tables staged here, copied with one H2D; results copied back here)
+  unsigned char* noalloc_host = nullptr;
 };
 
 extern "C" int qg_ctx_create(int device, void* stream, qg_ctx** out) {
@@ -139,9 +144,21 @@ extern "C" int qg_ctx_create(int device, void* stream, qg_ctx** out) {
   }
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
       set_pass_attributes() != hipSuccess || set_fused_attributes() != hipSuccess ||
-      fused_desc_size() != sizeof(FusedDescH)) {
+      fused_desc_size() != sizeof(FusedDescH) ||
+      hipHostMalloc((void**)&c->noalloc_host, kNoAllocScratchBytes, hipHostMallocDefault) != hipSuccess) {
+    if (c->noalloc_host) (void)hipHostFree(c->noalloc_host);
     delete c;
     return fail(QG_ERR_HIP, "qg_ctx_create: event creation or kernel attribute setup failed");
+  }
+  // the allocation-free entry point's device scratch: its address on this
+  // device, taken once per device with the device current
+  {
+    const hipError_t e = noalloc_scratch_init(device);
+    if (e != hipSuccess) {
+      (void)hipHostFree(c->noalloc_host);
+      delete c;
+      return fail(QG_ERR_HIP, std::string("qg_ctx_create: scratch address: ") + hipGetErrorString(e));
+    }
   }
   *out = c;
   return QG_OK;
@@ -154,6 +171,7 @@ extern "C" int qg_ctx_destroy(qg_ctx* c) {
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->noalloc_host) (void)hipHostFree(c->noalloc_host);
   delete c;
   return QG_OK;
 }
@@ -2080,6 +2098,123 @@ extern "C" int qg_mcclean_batch_grad_run(qg_plan* p, int B, const int32_t* axes,
   }
   p->last_ms = total_ms;
   p->last_state_ptr = nullptr;
+  return QG_OK;
+}
+
+// The whole-circuit McClean grad run of qg_mcclean_grad_run (n <= 13, no
+// ini_state), with every table and result in the device's static scratch
+// array instead of plan allocations.  The tables are built exactly as
+// mcclean_run builds them for the whole-circuit kernel, so the results are
+// bitwise the same.  No device or host allocation: the tables are staged in
+// the context's pinned mirror of the scratch, made at qg_ctx_create.
+extern "C" int qg_mcclean_grad_run_noalloc(qg_ctx* c, int n, int L, int entangler, const int32_t* axes,
+                                           const double* angles, const qg_term* terms, int nterms, double* E_out,
+                                           double* grad_out) {
+  if (!c || !axes || !angles || !E_out || !grad_out) return fail(QG_ERR_INVALID, "NULL argument");
+  if (n < 1 || n > kFusedMaxN)
+    return fail(QG_ERR_INVALID, "qg_mcclean_grad_run_noalloc: qubit number must be in [1, 13] (one workgroup per "
+                                "circuit); larger circuits need a plan and qg_mcclean_grad_run");
+  if (L < 1) return fail(QG_ERR_INVALID, "layer number must be >= 1");
+  if (entangler != QG_ENT_CNOT && entangler != QG_ENT_CZ) return fail(QG_ERR_INVALID, "unknown entangler");
+  if (nterms < 0 || (nterms > 0 && !terms)) return fail(QG_ERR_INVALID, "observable: bad term array");
+  for (int k = 0; k < L * n; ++k) {
+    if (axes[k] < 0 || axes[k] > 2) return fail(QG_ERR_INVALID, "Invalid axis " + std::to_string(axes[k]));
+    if (!std::isfinite(angles[k])) return fail(QG_ERR_INVALID, "non-finite angle");
+  }
+  // observable terms, validated as split_observable validates them
+  int ndiag = 0, nxy = 0;
+  for (int k = 0; k < nterms; ++k) {
+    const qg_term& t = terms[k];
+    if (t.i < 0 || t.i >= n) return fail(QG_ERR_INVALID, "observable: qubit index out of range");
+    if (!std::isfinite(t.w)) return fail(QG_ERR_INVALID, "observable: non-finite weight");
+    switch (t.kind) {
+      case QG_TERM_Z: ++ndiag; break;
+      case QG_TERM_ZZ:
+        if (t.j <= t.i || t.j >= n)
+          return fail(QG_ERR_INVALID, "observable: zz terms need i < j < n (upper triangle, observable.py:56-64)");
+        ++ndiag;
+        break;
+      case QG_TERM_X:
+      case QG_TERM_Y: ++nxy; break;
+      default: return fail(QG_ERR_INVALID, "observable: unknown term kind");
+    }
+  }
+  const int max_terms = std::max(64, n * n);  // as a McClean plan allows
+  if (ndiag > max_terms || nxy > max_terms) return fail(QG_ERR_INVALID, "too many observable terms");
+  // scratch layout (256-byte aligned pieces)
+  const size_t np = (size_t)L * n;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off = (off + bytes + 255) & ~(size_t)255;
+    return at;
+  };
+  const size_t o_desc = take(sizeof(FusedDescH));
+  const size_t o_type = take(sizeof(int32_t) * np);
+  const size_t o_gcs = take(sizeof(double) * 2 * np);
+  const size_t o_diag = take(sizeof(DTerm) * std::max(1, ndiag));
+  const size_t o_xy = take(sizeof(DTerm) * std::max(1, nxy));
+  const size_t o_map = take(sizeof(uint32_t) * n);
+  const size_t o_init = take(sizeof(double) * (n + 1));
+  const size_t o_out = take(sizeof(double) * (1 + np));
+  if (off > kNoAllocScratchBytes)
+    return fail(QG_ERR_INVALID, "qg_mcclean_grad_run_noalloc: the circuit's tables need " + std::to_string(off) +
+                                    " bytes, more than the " + std::to_string(kNoAllocScratchBytes) +
+                                    "-byte scratch (fewer layers, or a plan and qg_mcclean_grad_run)");
+  HIPCHK(hipSetDevice(c->device));
+  unsigned char* dev = (unsigned char*)noalloc_scratch(c->device);
+  if (!dev) return fail(QG_ERR_HIP, "qg_mcclean_grad_run_noalloc: no scratch address for this device");
+  unsigned char* h = c->noalloc_host;
+  // gates as (type, cos, sin), as mcclean_run's whole-circuit tables
+  int32_t* ht = (int32_t*)(h + o_type);
+  double* hg = (double*)(h + o_gcs);
+  for (size_t k = 0; k < np; ++k) {
+    ht[k] = axes[k] == 0 ? G_X : axes[k] == 1 ? G_Y : G_Z;
+    hg[2 * k] = std::cos(0.5 * angles[k]);
+    hg[2 * k + 1] = std::sin(0.5 * angles[k]);
+  }
+  // observable, in split_observable's order
+  DTerm* hd = (DTerm*)(h + o_diag);
+  DTerm* hx = (DTerm*)(h + o_xy);
+  int id = 0, ix = 0;
+  for (int k = 0; k < nterms; ++k) {
+    const qg_term& t = terms[k];
+    const int bi = n - 1 - t.i;
+    if (t.kind == QG_TERM_Z) hd[id++] = {bi, -1, t.w};
+    else if (t.kind == QG_TERM_ZZ) hd[id++] = {bi, n - 1 - t.j, t.w};
+    else if (t.kind == QG_TERM_X) hx[ix++] = {bi, 0, t.w};
+    else hx[ix++] = {bi, 1, t.w};
+  }
+  // the ladder's gather map and the RY(pi/4) table, as a McClean plan's
+  ladder_map(n, 1, (uint32_t*)(h + o_map));
+  ry_init_table(n, (double*)(h + o_init));
+  FusedDescH* hdsc = (FusedDescH*)(h + o_desc);
+  std::memset(hdsc, 0, sizeof(FusedDescH));
+  hdsc->n = n;
+  hdsc->L = L;
+  hdsc->kind = 0;
+  hdsc->ent = entangler;
+  hdsc->gtype = (const int32_t*)(dev + o_type);
+  hdsc->gcs = (const double*)(dev + o_gcs);
+  hdsc->dterms = (const DTerm*)(dev + o_diag);
+  hdsc->ndterms = ndiag;
+  hdsc->xy = (const DTerm*)(dev + o_xy);
+  hdsc->nxy = nxy;
+  hdsc->map = (const uint32_t*)(dev + o_map);
+  hdsc->init_tab = (const double*)(dev + o_init);
+  hdsc->out = (double*)(dev + o_out);
+  hdsc->grad = 1;
+  hdsc->psi_c64 = 0;
+  hdsc->psi_out = nullptr;
+  // tables in, one launch, results out (the kernel writes every result, so
+  // nothing in the scratch needs clearing)
+  HIPCHK(hipMemcpyAsync(dev, h, o_out, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(launch_fused(dev + o_desc, 1, n, c->stream));
+  HIPCHK(hipMemcpyAsync(h + o_out, dev + o_out, sizeof(double) * (1 + np), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const double* res = (const double*)(h + o_out);
+  *E_out = res[0];
+  std::memcpy(grad_out, res + 1, sizeof(double) * np);
   return QG_OK;
 }
 

@@ -18,6 +18,10 @@ constexpr int kMaxR = 5;           // log2 amplitudes per thread
 constexpr int kMaxSlots = kMaxStages * kMaxR;
 enum : int32_t { SL_ROT_A = 1, SL_ROT_B = 2, SL_RED = 4 };
 constexpr int kLutMax = 1024;      // phase-table entries staged in LDS (2W + 1 <= kLutMax)
+// Device scratch of the allocation-free whole-circuit entry point (one static
+// array per GPU; equals QG_NOALLOC_SCRATCH_BYTES in the public header).
+constexpr unsigned long kNoAllocScratchBytes = 256ul * 1024ul;
+constexpr int kNoAllocMaxDevices = 64;
 
 enum GateType : int32_t { G_NONE = 0, G_X = 1, G_Y = 2, G_Z = 3 };
 

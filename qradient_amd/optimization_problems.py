@@ -10,30 +10,33 @@ class MaxCut:
     3-regular graphs, the input family of the batched QAOA workload.
     """
 
+    # edge sources by precedence: an explicit set wins, then a random set of
+    # a fixed size, then a random set with a per-edge probability
+    _SOURCES = ('edge_set', 'edge_num', 'edge_probability')
+
     def __init__(self, vertex_num, **kwargs):
         self.vertex_num = vertex_num
-        if 'edge_set' in kwargs:
-            self.edge_set = kwargs['edge_set']
-        elif 'edge_num' in kwargs:
-            self.__random(edge_num=kwargs['edge_num'])
-        elif 'edge_probability' in kwargs:
-            self.__random(edge_probability=kwargs['edge_probability'])
-        else:
+        source = next((k for k in self._SOURCES if k in kwargs), None)
+        if source is None:
             raise ValueError('Specify one of the three edge_set, edge_num, or edge_probability')
+        self.edge_set = kwargs[source] if source == 'edge_set' else self._draw(source, kwargs[source])
 
-    def __random(self, **kwargs):
-        all_edges = np.array([[i, j] for i in range(self.vertex_num) for j in range(i + 1, self.vertex_num)])
-        if 'edge_num' in kwargs:
-            idx = np.random.choice(range(len(all_edges)), size=kwargs['edge_num'], replace=False)
-            self.edge_set = all_edges[idx]
-        else:
-            self.edge_set = np.array([e for e in all_edges if np.random.rand() < kwargs['edge_probability']])
+    def _draw(self, source, value):
+        """Random edges from the global numpy RNG.  Candidate pairs (i, j),
+        i < j, in row-major order; edge_num: one choice without replacement
+        over their indices; edge_probability: one uniform draw per pair, in
+        that order (the reference's draw sequence, so seeded graphs agree)."""
+        pairs = np.stack(np.triu_indices(self.vertex_num, 1), axis=1)
+        if source == 'edge_num':
+            return pairs[np.random.choice(range(len(pairs)), size=value, replace=False)]
+        keep = np.array([np.random.rand() < value for _ in range(len(pairs))], dtype=bool)
+        return pairs[keep]
 
     def to_observable(self):
         """{'zz': n x n object array with 1.0 at [a, b]} (optimization_problems.py:33-37)."""
-        zz = np.full([self.vertex_num, self.vertex_num], None)
-        for edge in self.edge_set:
-            zz[edge[0], edge[1]] = 1.
+        zz = np.full((self.vertex_num, self.vertex_num), None)
+        edges = np.asarray(self.edge_set, dtype=np.int64).reshape(-1, 2)
+        zz[edges[:, 0], edges[:, 1]] = 1.
         return {'zz': zz}
 
     @staticmethod

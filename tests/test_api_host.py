@@ -90,3 +90,30 @@ def test_maxcut_and_regular3():
     assert (deg == 3).all() and len(g.edge_set) == 30
     with pytest.raises(ValueError):
         MaxCut(4)
+
+
+def test_maxcut_random_edges_follow_the_reference_draw_sequence():
+    """optimization_problems.py:15-31: the candidate pairs come from the nested
+    loops i < j; edge_num picks indices with one np.random.choice call,
+    edge_probability draws one uniform per pair in that order.  Replayed here
+    by hand from the same seed."""
+    n = 7
+    pairs = [(i, j) for i in range(n) for j in range(i + 1, n)]
+    np.random.seed(5)
+    m = MaxCut(n, edge_num=6)
+    np.random.seed(5)
+    idx = np.random.choice(range(len(pairs)), size=6, replace=False)
+    assert [tuple(e) for e in m.edge_set] == [pairs[k] for k in idx]
+    np.random.seed(6)
+    m = MaxCut(n, edge_probability=0.4)
+    np.random.seed(6)
+    want = [p for p in pairs if np.random.rand() < 0.4]
+    assert [tuple(e) for e in m.edge_set] == want
+    # edge_set wins over the random sources, edge_num over edge_probability
+    assert MaxCut(3, edge_set=[[0, 2]], edge_num=1).edge_set == [[0, 2]]
+    np.random.seed(7)
+    a = MaxCut(n, edge_num=2, edge_probability=0.9).edge_set
+    np.random.seed(7)
+    assert np.array_equal(a, MaxCut(n, edge_num=2).edge_set)
+    zz = MaxCut(3, edge_set=np.zeros((0, 2), dtype=int)).to_observable()['zz']
+    assert all(v is None for v in zz.ravel())

@@ -283,9 +283,11 @@ int qg_plan_set_fused(qg_plan* plan, int on);
 int qg_plan_set_graph(qg_plan* plan, int on);
 /* LDS tile precision of a QG_C64 McClean plan: mode 1 keeps the pass tiles
  * in FP64 (the state is rounded to complex64 once per pass, when stored),
- * 0 in FP32 (also rounded at each of a pass's LDS transposes; faster), -1
- * (default) picks FP64 when the circuit has more than 200 passes.  The plan
- * is laid out again.  No counterpart in the reference (complex128 only). */
+ * 0 in FP32 (also rounded at each of a pass's LDS transposes; faster), 2
+ * FP64 in the forward passes and FP32 in the adjoint passes, -1 (default)
+ * picks 2 when the circuit has more than 200 passes, else 0.  The plan is
+ * laid out again.  qg_plan_lds64 returns the mode in effect (0, 1 or 2).
+ * No counterpart in the reference (complex128 only). */
 int qg_plan_set_lds64(qg_plan* plan, int mode);
 int qg_plan_lds64(const qg_plan* plan, int* out);
 /* The checkpoint period a McClean plan chose (0: full per-pass history). */

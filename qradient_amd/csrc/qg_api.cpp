@@ -376,8 +376,9 @@ struct qg_plan {
   int kmax = 13, cmin = 3;  // cmin 3: contiguous runs of >= 8 amplitudes in high-bit passes
   bool kmax_auto = true;    // kmax from default_kmax (not set by qg_plan_tune)
   int r_fwd = 4, r_rev = 4;  // amplitudes per thread (log2) of forward / history-adjoint passes (complex64)
-  int lds64_req = -1;  // complex64 McClean: FP64 LDS tiles (-1 auto, 0 no, 1 yes)
-  bool lds64 = false;
+  int lds64_req = -1;  // complex64 McClean: FP64 LDS tiles (-1 auto, 0 no, 1 all passes, 2 forward passes)
+  bool lds64 = false;      // forward passes use FP64 LDS tiles
+  bool lds64_adj = false;  // adjoint (two-state) passes too
   int chunk = 1;  // circuits resident at once (QAOA and McClean batches)
   int chunk_req = 0;  // QAOA: requested chunk (0: as many circuits as fit)
   int wg_per_cu = 16; // QAOA: workgroups per CU of a batched pass launch, in all circuits
@@ -487,7 +488,9 @@ static int ensure_workspace(qg_plan* p) {
 
 // Kernel storage code of a plan's passes: complex128 (0), complex64 (1), or
 // complex64 with FP64 LDS tiles (2).
-static int kdtype(const qg_plan* p) { return (p->dtype == QG_C64 && p->lds64) ? 2 : p->dtype; }
+static int kdtype(const qg_plan* p, bool two) {
+  return (p->dtype == QG_C64 && (two ? p->lds64_adj : p->lds64)) ? 2 : p->dtype;
+}
 
 static size_t lds_bytes_of(int dtype, int K, int threads) {  // the larger (adjoint) variant
   return (size_t)(1u << K) * (dtype == QG_C64 ? 8 : 16) + (size_t)((threads + 63) / 64) * (kMaxK + 1) * 8 +
@@ -505,10 +508,10 @@ constexpr int kWgOversub = 4;      // forward pass grids: this many times the re
 // estimate where the query fails).
 static int pass_occupancy(const qg_plan* p, const Geo& g, bool two, bool post, int phase) {
   int occ = 0;
-  if (launch_pass(kdtype(p), g.R, two, phase, post, nullptr, g.K, 0, 1, g.threads, p->n, nullptr, &occ, 0) !=
+  if (launch_pass(kdtype(p, two), g.R, two, phase, post, nullptr, g.K, 0, 1, g.threads, p->n, nullptr, &occ, 0) !=
           hipSuccess ||
       occ < 1) {
-    const size_t lds = lds_bytes_of(kdtype(p), g.K, g.threads);
+    const size_t lds = lds_bytes_of(kdtype(p, two), g.K, g.threads);
     occ = (int)std::max<size_t>(1, (160 * 1024) / lds);
     occ = std::min(occ, std::max(1, 512 / g.threads));
   }
@@ -974,7 +977,7 @@ static hipError_t launch_all(qg_plan* p) {
       if (e != hipSuccess) return e;
     }
     if (l.kind != 0) continue;
-    const hipError_t e = launch_pass(kdtype(p), l.R, l.two, l.phase, l.post, p->d_desc + l.desc, l.K, l.grid,
+    const hipError_t e = launch_pass(kdtype(p, l.two), l.R, l.two, l.phase, l.post, p->d_desc + l.desc, l.K, l.grid,
                                      l.batch, l.threads, p->n, c->stream, nullptr, l.tab);
     if (e != hipSuccess) return e;
   }
@@ -1026,7 +1029,7 @@ static int run_launches(qg_plan* p, int ndesc, int first_desc = 0) {
   if (p->use_graph && !p->profile) {
     std::vector<int64_t> sig;
     sig.reserve(p->launches.size() * 11 + 2);
-    sig.push_back(kdtype(p));
+    sig.push_back(kdtype(p, false) * 4 + kdtype(p, true));
     sig.push_back(p->n);
     sig.push_back((int64_t)(intptr_t)p->d_desc);
     // the elementwise observable launch bakes its buffers and term counts in
@@ -1080,7 +1083,7 @@ static int run_launches(qg_plan* p, int ndesc, int first_desc = 0) {
     const Launch& l = p->launches[k];
     if (pev) HIPCHK(hipEventRecord((*pev)[k], c->stream));
     if (l.kind == 0) {
-      HIPCHK(launch_pass(kdtype(p), l.R, l.two, l.phase, l.post, p->d_desc + l.desc, l.K, l.grid, l.batch,
+      HIPCHK(launch_pass(kdtype(p, l.two), l.R, l.two, l.phase, l.post, p->d_desc + l.desc, l.K, l.grid, l.batch,
                          l.threads, p->n, c->stream, nullptr, l.tab));
     } else if (l.kind == 1) {
       HIPCHK(launch_generic_obs(p));
@@ -1163,7 +1166,7 @@ static int stream_flush(qg_plan* p) {
   for (size_t k = g0; k < g1; ++k) {
     const Launch& l = p->launches[k];
     if (l.kind == 0) {
-      HIPCHK(launch_pass(kdtype(p), l.R, l.two, l.phase, l.post, p->d_desc + l.desc, l.K, l.grid, l.batch,
+      HIPCHK(launch_pass(kdtype(p, l.two), l.R, l.two, l.phase, l.post, p->d_desc + l.desc, l.K, l.grid, l.batch,
                          l.threads, p->n, c->stream, nullptr, l.tab));
     } else if (l.kind == 1) {
       HIPCHK(launch_generic_obs(p));
@@ -1292,6 +1295,13 @@ static void free_small(qg_plan* p) {
 // cannot then be allocated is an error (QG_ERR_NOMEM), not a different mode.
 constexpr double kPlanMemFraction = 0.75;
 
+// Whether the automatic FP64-tile rule (circuits of more than 200 passes)
+// covers the adjoint passes as well as the forward passes.  Measured at 30
+// qubits, 100 layers, checkpoints every 5 layers: FP64 tiles everywhere 5.78 s
+// and 6.2e-7 gradient error against complex128, forward only 5.44 s and 7.9e-7,
+// none 5.42 s and 1.1e-6.  The forward roundings dominate the error.
+constexpr bool kLds64Adjoint = false;
+
 static int plan_budget(qg_plan* p, size_t* budget) {
   *budget = (size_t)(kPlanMemFraction * (double)p->ctx->total_mem);
   return QG_OK;
@@ -1313,7 +1323,7 @@ static int mcclean_geometry(qg_plan* p) {
     // (1024-thread variants exist for those); otherwise fall back to 13
     const size_t S = (size_t)(1ull << n) * 8;
     const bool hist_fits = p->ckpt_req <= 0 && (size_t)(P * L + 4) * S <= budget;
-    const bool fp64_tiles = p->lds64_req == 1 || (p->lds64_req < 0 && P * L > 200);
+    const bool fp64_tiles = p->lds64_req >= 1 || (p->lds64_req < 0 && P * L > 200);
     if (p->dtype != QG_C64 || !hist_fits || fp64_tiles) {
       p->kmax = p->kmax_auto ? default_kmax(n, p->dtype, p->cmin, false) : 13;
       p->geo = layer_geometry(n, p->kmax, p->cmin);
@@ -1325,7 +1335,8 @@ static int mcclean_geometry(qg_plan* p) {
   // tiles; beyond about 200 passes these roundings would take the gradient
   // past a 1e-6 relative error, so long circuits use FP64 tiles (rounding
   // once per pass, at the store)
-  p->lds64 = p->dtype == QG_C64 && n >= 4 && (p->lds64_req == 1 || (p->lds64_req < 0 && P * L > 200));
+  p->lds64 = p->dtype == QG_C64 && n >= 4 && (p->lds64_req >= 1 || (p->lds64_req < 0 && P * L > 200));
+  p->lds64_adj = p->lds64 && (p->lds64_req == 1 || (p->lds64_req < 0 && kLds64Adjoint));
   const size_t hist_bytes = (size_t)(P * L + 4) * p->S;
   if (p->ckpt_req == QG_HISTORY && hist_bytes > budget)
     return fail(QG_ERR_NOMEM, "the per-pass history does not fit in device memory");
@@ -1870,6 +1881,7 @@ static int mcclean_batch_layout(qg_plan* p) {
   const int P = (int)p->geo.size();
   p->S = (size_t)(1ull << n) * (p->dtype == QG_C64 ? 8 : 16);
   p->lds64 = p->dtype == QG_C64 && n >= 4 && P * L > 200;
+  p->lds64_adj = p->lds64 && kLds64Adjoint;
   p->history = true;
   size_t budget = 0;
   CHK(plan_budget(p, &budget));
@@ -3235,7 +3247,7 @@ extern "C" int qg_plan_set_graph(qg_plan* p, int on) {
 
 extern "C" int qg_plan_set_lds64(qg_plan* p, int mode) {
   if (!p) return fail(QG_ERR_INVALID, "NULL plan");
-  if (mode < -1 || mode > 1) return fail(QG_ERR_INVALID, "mode must be -1, 0 or 1");
+  if (mode < -1 || mode > 2) return fail(QG_ERR_INVALID, "mode must be -1, 0, 1 or 2");
   if (p->mc_batch) return fail(QG_ERR_UNSUPPORTED, "batch plans keep the default tile shape");
   if (p->kind != PK_MCCLEAN) return fail(QG_ERR_UNSUPPORTED, "FP64 tiles apply to McClean plans");
   p->lds64_req = mode;
@@ -3244,7 +3256,7 @@ extern "C" int qg_plan_set_lds64(qg_plan* p, int mode) {
 
 extern "C" int qg_plan_lds64(const qg_plan* p, int* out) {
   if (!p || !out) return fail(QG_ERR_INVALID, "NULL argument");
-  *out = p->lds64 ? 1 : 0;
+  *out = !p->lds64 ? 0 : p->lds64_adj ? 1 : 2;
   return QG_OK;
 }
 

@@ -280,6 +280,10 @@ def bench_large(args, sync):
         if k_tuned is None:
             k_tuned, _ = autotune.tune_ckpt(n, L, dtype)
         circ = McClean(n, obs, L, axes=axes, angles=angles, dtype=dtype, ckpt_every=int(k_tuned))
+        # the tuned tile shape of this shape (cached or committed), else the library default
+        shape = autotune.cached('mcclean', n, L, dtype, 'cnot')
+        if shape is not None:
+            autotune.apply(circ._get_plan(), shape)
         circ.grad_run()
         sync()
         t0 = time.perf_counter()
@@ -290,7 +294,7 @@ def bench_large(args, sync):
         _native.call('qg_plan_ckpt_every', circ._plan, ctypes.byref(k))
         t64 = ctypes.c_int32()
         _native.call('qg_plan_lds64', circ._plan, ctypes.byref(t64))
-        res[dtype] = dict(ms=dt * 1e3, E=E, g=g, k=int(k.value), lds64=int(t64.value))
+        res[dtype] = dict(ms=dt * 1e3, E=E, g=g, k=int(k.value), lds64=int(t64.value), shape=shape)
         circ._destroy_plan()
     c64, c128 = res['complex64'], res['complex128']
     rel = float(np.linalg.norm(c64['g'] - c128['g']) / np.linalg.norm(c128['g']))
@@ -300,8 +304,9 @@ def bench_large(args, sync):
                  'checkpoints every %d layers)' % c64['k'], 1e3 / c64['ms'], 'circuits/s', 1, 0, c64['ms'],
                  'c64 (FP64 arithmetic and reductions)',
                  {'workload': 'mcclean_grad_run_n30_L100_zzchain', 'qubits': n, 'layers': L, 'seed': SEED,
-                  'ckpt_every': c64['k'], 'ckpt_every_tuned': True, 'fp64_lds_tiles': bool(c64['lds64'])},
-                 c128_plain={'ms_per_step': c128['ms'], 'ckpt_every': c128['k']},
+                  'ckpt_every': c64['k'], 'ckpt_every_tuned': True, 'lds64_mode': c64['lds64'],
+                  'tile_shape': c64['shape']},
+                 c128_plain={'ms_per_step': c128['ms'], 'ckpt_every': c128['k'], 'tile_shape': c128['shape']},
                  c64_vs_c128_grad_rel_err=rel, effective_gbs=b_alg / (c64['ms'] * 1e-3) / 1e9, b_alg_bytes=b_alg)
 
 

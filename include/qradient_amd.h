@@ -268,7 +268,8 @@ int qg_plan_pass_times(const qg_plan* plan, int cap, float* ms, int* kinds, int*
  * again (its pass count, memory mode and workspace size may change; a
  * caller-supplied workspace that is too small is an error).  Used by the
  * autotuner.  Default: the fewest passes per layer that 13 local bits allow,
- * with the smallest kmax (>= 11 for QG_C64, >= 12 for QG_C128) that keeps it. */
+ * with the smallest kmax (>= 11 for QG_C64, >= 12 for QG_C128) that keeps it;
+ * kmax 13 for QG_C64 states of 4 GiB and more (n >= 29). */
 int qg_plan_tune(qg_plan* plan, int kmax, int cmin, int r_fwd, int r_rev);
 /* on = 1 (default): circuits with n <= kmax run whole in one workgroup each
  * (one launch per call, FP64 throughout); on = 0 uses the pass kernels. */

@@ -36,7 +36,10 @@ def _passes(n, kmax, cmin):
 def default_shape(n, dtype):
     """The library's untuned shape: the fewest passes per layer 13 local bits
     allow, with the smallest tile (>= 11 bits for complex64, 12 for
-    complex128) that keeps that count."""
+    complex128) that keeps that count; 13 bits for complex64 states of 4 GiB
+    and more (n >= 29)."""
+    if dtype == 'complex64' and n >= 29:
+        return dict(DEFAULT, kmax=13)
     best = _passes(n, 13, 3)
     for kmax in range(11 if dtype == 'complex64' else 12, 13):
         if _passes(n, kmax, 3) == best:

@@ -342,7 +342,9 @@ int qg_state_copy(qg_state* dst, const qg_state* src);
  * and a shape change allocates nothing once the plans have grown.  A request
  * is admitted when a worker is idle and the arena can hold its workspace
  * (qg_server_footprint); requests that fit may overtake a head-of-queue
- * request that does not at most backfill_limit times.  A request's memory
+ * request that does not at most backfill_limit times (with
+ * QG_SERVER_SHORTEST_FIRST: a request may be overtaken by later arrivals at
+ * most backfill_limit times).  A request's memory
  * mode is a function of its shape only: the per-pass history when that
  * needs at most history_max_bytes, else checkpoints every ckpt_every layers,
  * so its result is bitwise the same alone and in any mix.  The observable is
@@ -350,6 +352,10 @@ int qg_state_copy(qg_state* dst, const qg_state* src);
  * server functions may be called from any thread. */
 #define QG_SERVER_SIMULATE 1 /* flags: no device; a request sleeps ~1 us per MiB of
                                 workspace and returns E = n, grad = L (scheduler tests) */
+#define QG_SERVER_SHORTEST_FIRST 2 /* flags: admit the cheapest queued request that fits
+                                      (estimated work (6 L + 3) 2^n) instead of the oldest;
+                                      a request overtaken backfill_limit times by later
+                                      arrivals is served next, in arrival order */
 typedef struct qg_server qg_server;
 typedef struct qg_request_info {
   double queue_ms;        /* submit -> start on a worker (host clock) */

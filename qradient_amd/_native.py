@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'libqradient
 
 QG_OK, QG_ERR_INVALID, QG_ERR_UNSUPPORTED, QG_ERR_HIP, QG_ERR_NOMEM, QG_ERR_PENDING = 0, 1, 2, 3, 4, 5
 QG_SERVER_SIMULATE = 1
+QG_SERVER_SHORTEST_FIRST = 2
 QG_C128, QG_C64 = 0, 1
 QG_ENT_CNOT, QG_ENT_CZ = 0, 1
 QG_HISTORY = -1

@@ -17,6 +17,12 @@
 // every ckpt_every layers), so a request's result is bitwise the same alone
 // and in any mix.
 //
+// QG_SERVER_SHORTEST_FIRST replaces the arrival order by the estimated work
+// of a request (its algorithmic bytes, (6 L + 3) 2^n amplitudes): the cheapest
+// queued request that fits starts first.  A request that later arrivals have
+// overtaken backfill_limit times is served in arrival order before any other,
+// and while it does not fit nothing else starts (aging: no starvation).
+//
 // QG_SERVER_SIMULATE runs the same admission and scheduling without a device:
 // a request sleeps for a time proportional to its footprint and returns
 // E = n and grad = L everywhere.  It exists so the scheduler can be tested
@@ -113,6 +119,8 @@ struct Req {
   int ckpt = QG_HISTORY;  // the plan's ckpt_every argument
   int mode = 0;           // reported: 0 history, k > 0 checkpoints
   size_t footprint = 0, offset = 0;
+  double cost = 0.0;   // estimated work: (6 L + 3) 2^n, in amplitudes moved
+  int overtaken = 0;   // later arrivals started before this request
   bool done = false;
   int status = QG_OK;
   std::string err;
@@ -152,6 +160,7 @@ struct qg_server {
   bool closing = false;
   uint64_t completed = 0;
   bool simulate() const { return (flags & QG_SERVER_SIMULATE) != 0; }
+  bool shortest_first() const { return (flags & QG_SERVER_SHORTEST_FIRST) != 0; }
 };
 
 namespace {
@@ -173,18 +182,43 @@ int footprint_of(qg_server* s, int n, int L, int* ckpt, int* mode, size_t* bytes
   return QG_OK;
 }
 
-// Index of the queued request to run next, its arena block taken; -1 if none
-// may run now.  (Lock held.)
-int pick(qg_server* s) {
+// Shortest-first order: the oldest request that has been overtaken
+// backfill_limit times, else the cheapest that fits (ties: the earlier).
+int pick_shortest(qg_server* s) {
   for (size_t i = 0; i < s->queue.size(); ++i) {
     Req* r = s->queue[i];
-    if (s->arena.alloc(r->footprint, &r->offset)) {
-      s->bypassed = i > 0 ? s->bypassed + 1 : 0;
-      return (int)i;
-    }
-    if (i == 0 && s->bypassed >= s->backfill_limit) return -1;  // hold capacity for the head
+    if (r->overtaken < s->backfill_limit) continue;
+    return s->arena.alloc(r->footprint, &r->offset) ? (int)i : -1;  // else hold capacity for it
   }
+  std::vector<int> order(s->queue.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return s->queue[a]->cost < s->queue[b]->cost; });
+  for (int i : order)
+    if (s->arena.alloc(s->queue[i]->footprint, &s->queue[i]->offset)) return i;
   return -1;
+}
+
+// Index of the queued request to run next, its arena block taken; -1 if none
+// may run now.  The requests queued before it count one more overtaking.
+// (Lock held.)
+int pick(qg_server* s) {
+  int pick_i = -1;
+  if (s->shortest_first()) {
+    pick_i = pick_shortest(s);
+  } else {
+    for (size_t i = 0; i < s->queue.size(); ++i) {
+      Req* r = s->queue[i];
+      if (s->arena.alloc(r->footprint, &r->offset)) {
+        s->bypassed = i > 0 ? s->bypassed + 1 : 0;
+        pick_i = (int)i;
+        break;
+      }
+      if (i == 0 && s->bypassed >= s->backfill_limit) return -1;  // hold capacity for the head
+    }
+  }
+  for (int i = 0; i < pick_i; ++i) s->queue[i]->overtaken++;
+  return pick_i;
 }
 
 int run_on_device(qg_server* s, Slot& sl, Req* r) {
@@ -361,6 +395,7 @@ extern "C" int qg_submit(qg_server* s, int n, int L, const int32_t* axes, const 
   r->L = L;
   rc = footprint_of(s, n, L, &r->ckpt, &r->mode, &r->footprint);
   if (rc != QG_OK) return rc;
+  r->cost = (6.0 * L + 3.0) * std::ldexp(1.0, n);
   if (Arena::round(r->footprint) > s->arena.size())
     return qg::set_error(QG_ERR_NOMEM, "request needs " + std::to_string(r->footprint) +
                                            " bytes of workspace, the server's arena holds " +

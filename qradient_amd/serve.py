@@ -46,12 +46,18 @@ def _info_dict(info):
 class GradServer:
     """``budget_bytes``: the device arena (default 80% of the free memory);
     ``history_max_bytes`` / ``ckpt_every``: the shape-to-memory-mode rule;
-    ``backfill_limit``: how often a head-of-queue request may be overtaken."""
+    ``policy``: 'fifo' admits the oldest queued request that fits, and
+    ``backfill_limit`` bounds how often a head-of-queue request that does not
+    fit may be overtaken; 'shortest' admits the cheapest request that fits
+    (estimated work (6 L + 3) 2^n), and a request overtaken ``backfill_limit``
+    times by later arrivals is served next."""
 
     def __init__(self, device=0, slots=4, budget_bytes=None, dtype='complex64', history_max_bytes=4 << 30,
-                 ckpt_every=8, backfill_limit=8, simulate=False):
+                 ckpt_every=8, backfill_limit=8, simulate=False, policy='fifo'):
         if dtype not in _native.DTYPES:
             raise ValueError('dtype must be complex128 or complex64')
+        if policy not in ('fifo', 'shortest'):
+            raise ValueError("policy must be 'fifo' or 'shortest'")
         if budget_bytes is None:
             if simulate:
                 budget_bytes = 1 << 40
@@ -65,7 +71,8 @@ class GradServer:
         h = ctypes.c_void_p()
         _native.call('qg_server_create', int(device), self.nslots, _native.DTYPES[dtype], self.budget,
                      int(history_max_bytes), int(ckpt_every), int(backfill_limit),
-                     _native.QG_SERVER_SIMULATE if simulate else 0, ctypes.byref(h))
+                     (_native.QG_SERVER_SIMULATE if simulate else 0) |
+                     (_native.QG_SERVER_SHORTEST_FIRST if policy == 'shortest' else 0), ctypes.byref(h))
         self._h = h
         self._lock = threading.Lock()
         self._pending = {}        # ticket -> Request
@@ -221,9 +228,10 @@ def main(argv=None):
     ap.add_argument('--lmin', type=int, default=4)
     ap.add_argument('--lmax', type=int, default=64)
     ap.add_argument('--seed', type=int, default=0)
+    ap.add_argument('--policy', choices=['fifo', 'shortest'], default='shortest')
     a = ap.parse_args(argv)
     trace = poisson_trace(a.requests, a.rate, a.seed, (a.nmin, a.nmax), (a.lmin, a.lmax))
-    st = replay(trace, slots=a.slots)
+    st = replay(trace, slots=a.slots, policy=a.policy)
     st.update({'rate_per_s': a.rate, 'n_range': [a.nmin, a.nmax], 'L_range': [a.lmin, a.lmax],
                'dtype': 'complex64 state, FP64 arithmetic and reductions', 'slots': a.slots})
     print(json.dumps(st))

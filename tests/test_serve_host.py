@@ -91,3 +91,44 @@ def test_poisson_trace_is_seeded():
     assert [x[:3] for x in a] == [x[:3] for x in b]
     assert all(12 <= x[1] <= 28 and 4 <= x[2] <= 64 for x in a)
     assert all(a[i][0] < a[i + 1][0] for i in range(49))
+
+
+def _order(srv, reqs):
+    """Completion order when reqs[0] is running (its simulated run takes about
+    1 us per MiB of workspace: tens of ms) while the others are queued."""
+    import time
+    futs = [srv.submit(reqs[0])]
+    t_end = time.time() + 10
+    while srv.counters()['running'] < 1 and not futs[0].done():
+        assert time.time() < t_end
+        time.sleep(1e-4)
+    futs += [srv.submit(r) for r in reqs[1:]]
+    for f in futs:
+        f.result(timeout=30)
+    order = [r.tag for r in srv.completed]
+    srv.close()
+    return order
+
+
+def test_shortest_first_runs_the_cheapest_queued_request():
+    """One slot: while the first request runs the others queue; they start by
+    estimated work (6 L + 3) 2^n, not by arrival."""
+    srv = GradServer(slots=1, budget_bytes=1 << 40, history_max_bytes=1 << 40, simulate=True, policy='shortest')
+    reqs = [_req(26, 16, tag='first'), _req(24, 20, tag='big'), _req(16, 4, tag='small'), _req(20, 6, tag='medium')]
+    assert _order(srv, reqs) == ['first', 'small', 'medium', 'big']
+
+
+def test_shortest_first_ages_an_overtaken_request():
+    """A request overtaken backfill_limit times by later arrivals is served next."""
+    srv = GradServer(slots=1, budget_bytes=1 << 40, history_max_bytes=1 << 40, backfill_limit=2, simulate=True,
+                     policy='shortest')
+    reqs = [_req(26, 16, tag='first'), _req(24, 20, tag='big')] + [_req(16, 4, tag='s%d' % k) for k in range(5)]
+    assert _order(srv, reqs) == ['first', 's0', 's1', 'big', 's2', 's3', 's4']
+
+
+def test_fifo_keeps_arrival_order_on_one_slot():
+    srv = GradServer(slots=1, budget_bytes=1 << 40, history_max_bytes=1 << 40, simulate=True)
+    reqs = [_req(26, 16, tag='first'), _req(24, 20, tag='big'), _req(16, 4, tag='small'), _req(20, 6, tag='medium')]
+    assert _order(srv, reqs) == ['first', 'big', 'small', 'medium']
+    with pytest.raises(ValueError):
+        GradServer(simulate=True, policy='random')

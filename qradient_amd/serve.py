@@ -222,16 +222,17 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description='Poisson-arrival grad_run serving benchmark')
     ap.add_argument('--requests', type=int, default=200)
     ap.add_argument('--rate', type=float, default=20.0, help='arrivals per second')
-    ap.add_argument('--slots', type=int, default=8)
+    ap.add_argument('--slots', type=int, default=4)
     ap.add_argument('--nmin', type=int, default=12)
     ap.add_argument('--nmax', type=int, default=28)
     ap.add_argument('--lmin', type=int, default=4)
     ap.add_argument('--lmax', type=int, default=64)
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--policy', choices=['fifo', 'shortest'], default='shortest')
+    ap.add_argument('--backfill-limit', type=int, default=32)
     a = ap.parse_args(argv)
     trace = poisson_trace(a.requests, a.rate, a.seed, (a.nmin, a.nmax), (a.lmin, a.lmax))
-    st = replay(trace, slots=a.slots, policy=a.policy)
+    st = replay(trace, slots=a.slots, policy=a.policy, backfill_limit=a.backfill_limit)
     st.update({'rate_per_s': a.rate, 'n_range': [a.nmin, a.nmax], 'L_range': [a.lmin, a.lmax],
                'dtype': 'complex64 state, FP64 arithmetic and reductions', 'slots': a.slots})
     print(json.dumps(st))

@@ -313,13 +313,15 @@ def bench_large(args, sync):
     rel = float(np.linalg.norm(c64['g'] - c128['g']) / np.linalg.norm(c128['g']))
     sb = (2 ** n) * 8
     b_alg = (6 * L + 3 + 2 * -(-L // c64['k'])) * sb
+    b_alg128 = (6 * L + 3 + 2 * -(-L // c128['k'])) * 2 * sb
     return _line('grad_run circuits/s (McClean 30 qubits, 100 layers, complex64 state + FP64 reductions, '
                  'checkpoints every %d layers)' % c64['k'], 1e3 / c64['ms'], 'circuits/s', 1, 0, c64['ms'],
                  'c64 (FP64 arithmetic and reductions)',
                  {'workload': 'mcclean_grad_run_n30_L100_zzchain', 'qubits': n, 'layers': L, 'seed': SEED,
                   'ckpt_every': c64['k'], 'ckpt_every_tuned': True, 'lds64_mode': c64['lds64'],
                   'tile_shape': c64['shape']},
-                 c128_plain={'ms_per_step': c128['ms'], 'ckpt_every': c128['k'], 'tile_shape': c128['shape']},
+                 c128_plain={'ms_per_step': c128['ms'], 'ckpt_every': c128['k'], 'tile_shape': c128['shape'],
+                             'effective_gbs': b_alg128 / (c128['ms'] * 1e-3) / 1e9, 'b_alg_bytes': b_alg128},
                  c64_vs_c128_grad_rel_err=rel, effective_gbs=b_alg / (c64['ms'] * 1e-3) / 1e9, b_alg_bytes=b_alg)
 
 

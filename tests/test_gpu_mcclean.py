@@ -261,6 +261,39 @@ def test_states_of_4_gib_and_more(gpu, n, lds64):
     assert np.all(np.isfinite(g1)) and np.linalg.norm(g1) > 0
 
 
+def test_largest_complex64_state(gpu):
+    """31 qubits in complex64 (16 GiB per state, 2^31 amplitudes: the largest
+    state whose plan fits the device's budget), 2 layers.  No oracle and no
+    complex128 run fit at this size, so the checks are what the mathematics
+    fixes: the exact +-pi/2 parameter-shift rule (mc_clean.py:139-155) on one
+    parameter in each pass's bit range, from complex64 forward runs, and the
+    analytic zeros of the last layer's Z-axis gradients under a diagonal
+    observable."""
+    from qradient_amd.circuit_logic import McClean
+    n, L = 31, 2
+    axes, angles, obs = _case(n, L, 131)
+    axes[L - 1, :4] = 2  # Z-axis rotations on the last layer: zero gradients
+    c = McClean(n, obs, L, axes=axes, angles=angles.copy(), dtype='complex64')
+    print('largest state: n=31 L=2 complex64 grad_run', flush=True)
+    E, g = c.grad_run()
+    gmax = float(np.max(np.abs(g)))
+    assert gmax > 0 and np.all(np.isfinite(g))
+    assert np.max(np.abs(g[L - 1][axes[L - 1] == 2])) <= 1e-6 * gmax
+    # complex64 expectations carry about 1e-7 of the observable's weight
+    scale = max(gmax, sum(abs(float(w)) for w in obs['zz'].ravel() if w is not None))
+    for (i, q) in [(0, 30), (0, 15), (0, 2), (1, 20)]:   # bits 0, 15, 28 and 10
+        base = c.angles[i, q]
+        vals = []
+        for s in (np.pi / 2, -np.pi / 2):
+            c.angles[i, q] = base + s
+            vals.append(c.run_expec_val())
+        c.angles[i, q] = base
+        shift = 0.5 * (vals[0] - vals[1])
+        print('  shift check ({}, {}): adjoint {:.6e} shift {:.6e}'.format(i, q, g[i, q], shift), flush=True)
+        assert abs(shift - g[i, q]) <= 1e-5 * scale, (i, q, shift, g[i, q])
+    c._destroy_plan()
+
+
 def test_plan_mode_and_bits_do_not_depend_on_free_memory(gpu):
     """The planner picks history or checkpoints from the circuit shape and the
     device's total memory.  A 26-qubit, 64-layer complex64 circuit keeps its

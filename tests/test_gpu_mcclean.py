@@ -269,7 +269,11 @@ def test_largest_complex64_state(gpu):
     parameter in each pass's bit range, from complex64 forward runs, and the
     analytic zeros of the last layer's Z-axis gradients under a diagonal
     observable."""
+    import gc
+    import torch
     from qradient_amd.circuit_logic import McClean
+    gc.collect()               # plans of earlier tests' circuits
+    torch.cuda.empty_cache()   # blocks torch keeps reserved
     n, L = 31, 2
     axes, angles, obs = _case(n, L, 131)
     axes[L - 1, :4] = 2  # Z-axis rotations on the last layer: zero gradients

@@ -167,6 +167,37 @@ def _line(metric, value, unit, steps, warmup, ms, dtype, config, **extra):
     return out
 
 
+def _qaoa_cpu_job(job):
+    from oracle import c_oracle
+    from oracle import statevector as sv
+    n, edges, betas, gammas = job
+    c_oracle.qaoa_grad_run(n, sv.maxcut_observable(n, edges), betas, gammas)
+    return 0
+
+
+def qaoa_cpu_baseline(wl, n, count=32):
+    """The C restatement of the reference algorithm (oracle/c/oracle_ref.c,
+    qaoa.py:40-70 with stored history) over the first ``count`` circuits of
+    the workload: independent circuits, one per process, single-threaded each,
+    on up to 16 host cores.  Runs before any device work (the workers are
+    forked)."""
+    import multiprocessing as mp
+    from threadpoolctl import threadpool_limits
+    from oracle import c_oracle
+    c_oracle.load()  # loaded once, inherited by the workers
+    P = min(16, len(os.sched_getaffinity(0)))
+    jobs = [(n, wl['edges'][b], wl['betas'][b], wl['gammas'][b]) for b in range(count)]
+    with threadpool_limits(1):
+        with mp.get_context('fork').Pool(P) as pool:
+            t0 = time.perf_counter()
+            pool.map(_qaoa_cpu_job, jobs, chunksize=1)
+            dt = time.perf_counter() - t0
+    return {'value': count / dt, 'unit': 'circuits/s', 'cores': P, 'kind': 'port',
+            'sample': 'first {} of the 512 circuits (20 qubits, p = 8), C restatement oracle/c/oracle_ref.c, one '
+                      'circuit per process on {} processes of 1 thread; {:.2f} s wall; host has {} cores'.format(
+                          count, P, dt, len(os.sched_getaffinity(0)))}
+
+
 def bench_qaoa(args, sync):
     """512 seeded random 3-regular MaxCut graphs, 20 qubits, p = 8: one batched grad_run per step."""
     from qradient_amd.batch import QaoaBatch
@@ -175,6 +206,7 @@ def bench_qaoa(args, sync):
     # the workload's graphs and angles, committed as a fixture (written by
     # tests/golden/make_golden.py with the library's seeded generator)
     wl = np.load(os.path.join(ROOT, 'tests', 'golden', 'qaoa_reg3_n20_b512.npz'), allow_pickle=False)
+    cpu = None if args.no_cpu_baseline else qaoa_cpu_baseline(wl, n)
     obs = [MaxCut(n, edge_set=wl['edges'][b]).to_observable() for b in range(B)]
     gammas = wl['gammas']
     betas = wl['betas']
@@ -215,7 +247,7 @@ def bench_qaoa(args, sync):
                  c64_vs_c128_grad_rel_err_max=float(rel.max()),
                  grad_rel_err_vs_oracle=qaoa_oracle_errors(res),
                  effective_gbs=b_alg / (c64['dev_ms'] * 1e-3) / 1e9,
-                 b_alg_bytes=b_alg)
+                 b_alg_bytes=b_alg, **({'cpu_baseline': cpu} if cpu else {}))
 
 
 def qaoa_oracle_errors(res):

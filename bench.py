@@ -247,7 +247,21 @@ def bench_qaoa(args, sync):
                  c64_vs_c128_grad_rel_err_max=float(rel.max()),
                  grad_rel_err_vs_oracle=qaoa_oracle_errors(res),
                  effective_gbs=b_alg / (c64['dev_ms'] * 1e-3) / 1e9,
-                 b_alg_bytes=b_alg, **({'cpu_baseline': cpu} if cpu else {}))
+                 b_alg_bytes=b_alg, hbm=qaoa_traffic(c64['dev_ms']), **({'cpu_baseline': cpu} if cpu else {}))
+
+
+def qaoa_traffic(dev_ms):
+    """Bytes the batch's passes move (committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    counts of this workload, not measured in this run) over the device time."""
+    path = os.path.join(ROOT, 'profiles', 'traffic_qaoa_n20_p8_b512.json')
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    gbs = t['c64_batch_bytes'] / (dev_ms * 1e-3) / 1e9
+    return {'traffic_bytes_per_batch': t['c64_batch_bytes'], 'sustained_gbs': gbs, 'sustained_frac': gbs / HBM_PEAK_GBS,
+            'traffic_source': 'profiles/traffic_qaoa_n20_p8_b512.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of '
+                              'this workload, scaled by a copy-kernel calibration; committed, not measured in this run'}
 
 
 def qaoa_oracle_errors(res):

@@ -14,7 +14,8 @@ def _reqs():
     return poisson_trace(12, 50.0, seed=5, n_range=(8, 18), L_range=(3, 12))
 
 
-def test_results_identical_alone_and_in_mix(gpu):
+@pytest.mark.parametrize('policy', ['fifo', 'shortest'])
+def test_results_identical_alone_and_in_mix(gpu, policy):
     from qradient_amd.serve import GradServer, Request
     trace = _reqs()
     alone = []
@@ -22,7 +23,7 @@ def test_results_identical_alone_and_in_mix(gpu):
     for _, n, L, axes, angles, obs in trace:
         alone.append(srv.submit(Request(n, L, axes, angles, obs)).result(timeout=120))
     srv.close()
-    srv = GradServer(slots=6, budget_bytes=8 << 30)
+    srv = GradServer(slots=6, budget_bytes=8 << 30, policy=policy)
     futs = [srv.submit(Request(n, L, axes, angles, obs)) for _, n, L, axes, angles, obs in trace]
     mixed = [f.result(timeout=120) for f in futs]
     st = srv.stats()

@@ -687,7 +687,7 @@ extern "C" int qg_plan_last_ms(const qg_plan* p, double* ms) {
 // prob[k] = <v|(I + P_k)/2|v> = |P_k^+ v|^2 for each term, in the given order
 // (weights ignored): the probability of the +1 outcome that the projectors of
 // observable.py:126-177 give in base.py:22-46, also for a state whose norm is
-// not exactly 1.  One pass over v per kTermsPerLaunch terms.
+// not exactly 1.  One pass over v per 32 diagonal or 16 other terms.
 // The term table of the term-expectation kernel: terms[0, nterms) in order,
 // then the identity (|v|^2).
 static int pterm_table(int n, const qg_term* terms, int nterms, std::vector<PTerm>& h) {
@@ -719,14 +719,14 @@ static int pterm_table(int n, const qg_term* terms, int nterms, std::vector<PTer
 }
 
 // e[k] = <v|P_k|v> for the terms in order, e[nterms] = |v|^2 (weights
-// ignored).  One pass over v per kTermsPerLaunch terms.
+// ignored).  One pass over v per 32 diagonal or 16 other terms.
 static int term_expec(qg_ctx* c, int dtype, const void* v, int n, const qg_term* terms, int nterms, double* e) {
   std::vector<PTerm> h;
   CHK(pterm_table(n, terms, nterms, h));
   const int nt = nterms + 1;
   HIPCHK(hipSetDevice(c->device));
   const size_t tb = ((sizeof(PTerm) * nt + 255) / 256) * 256;
-  const size_t pb = sizeof(double) * kTermBlocks * kTermsPerLaunch;
+  const size_t pb = sizeof(double) * kTermBlocks * kTermPartRows;
   char* buf = nullptr;
   HIPCHK(hipMalloc(&buf, tb + pb + sizeof(double) * nt));
   PTerm* d_t = (PTerm*)buf;
@@ -2397,7 +2397,7 @@ extern "C" int qg_mcclean_shift_term_prob(qg_plan* p, const int32_t* axes, const
   CHK(pterm_table(p->n, terms, nterms, h));
   const int nt = nterms + 1;
   const size_t tb = ((sizeof(PTerm) * nt + 255) / 256) * 256;
-  const size_t pb = sizeof(double) * kTermBlocks * kTermsPerLaunch;
+  const size_t pb = sizeof(double) * kTermBlocks * kTermPartRows;
   HIPCHK(hipSetDevice(p->ctx->device));
   DevBuf db;
   HIPCHK(hipMalloc(&db.p, tb + pb + sizeof(double) * (size_t)nt * std::max(nshift, 1)));

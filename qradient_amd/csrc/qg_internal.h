@@ -61,7 +61,9 @@ struct DTerm {
 struct PTerm {
   int32_t kind, a, b, pad;
 };
-constexpr int kTermsPerLaunch = 16;  // terms measured per pass over the state
+constexpr int kTermsPerLaunch = 16;  // terms measured per pass over the state (general kernel)
+constexpr int kDiagTermsPerLaunch = 24;  // diagonal (Z / ZZ / identity) terms per pass of the mask kernel
+constexpr int kTermPartRows = kDiagTermsPerLaunch;  // partial rows a term launch may write
 constexpr int kTermBlocks = 2048;    // workgroups of a term pass (partials per term)
 constexpr int kSampleBlock = 1024;   // positions per block of the basis sampler
 

@@ -190,20 +190,25 @@ __global__ void __launch_bounds__(256) k_term_expec(const typename V2<S>::T* __r
 // and (-1)^popc(j & (m_k >> sb)), the same for the whole grid at iteration j.
 // Each thread sums (-1)^popc(j & mh_k) |v_x|^2 over its iterations (one FMA
 // per term and amplitude) and applies its own sign at the end.
+// Up to kDiagTermsPerLaunch terms per launch: the per-thread sign of a term
+// is recomputed from its mask after the loop, so a term costs one FP64
+// accumulator in the loop.
 template <class S>
 __global__ void __launch_bounds__(256) k_term_expec_diag(const typename V2<S>::T* __restrict__ v, uint32_t N,
                                                          const PTerm* __restrict__ t, int nt, int sb,
                                                          double* __restrict__ part) {
-  __shared__ double red[4][kTermsPerLaunch];
+  constexpr int NT = kDiagTermsPerLaunch;
+  __shared__ double red[4][NT];
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t mh[kTermsPerLaunch];
-  double acc[kTermsPerLaunch], sl[kTermsPerLaunch];
+  auto mask_of = [&](int k) -> uint32_t {
+    return (k < nt && t[k].kind == 0) ? (1u << t[k].a) | (t[k].b >= 0 ? (1u << t[k].b) : 0u) : 0u;
+  };
+  uint32_t mh[NT];
+  double acc[NT];
 #pragma unroll
-  for (int k = 0; k < kTermsPerLaunch; ++k) {
-    uint32_t m = 0u;
-    if (k < nt && t[k].kind == 0) m = (1u << t[k].a) | (t[k].b >= 0 ? (1u << t[k].b) : 0u);
+  for (int k = 0; k < NT; ++k) {
+    const uint32_t m = mask_of(k);
     mh[k] = sb < 32 ? m >> sb : 0u;
-    sl[k] = (__popc(g & m) & 1u) ? -1.0 : 1.0;
     acc[k] = 0.0;
   }
   const uint32_t iters = N >> sb;  // N and the stride are powers of two
@@ -216,7 +221,7 @@ __global__ void __launch_bounds__(256) k_term_expec_diag(const typename V2<S>::T
       p1 = (double)a1.x * a1.x + (double)a1.y * a1.y;
     }
 #pragma unroll
-    for (int k = 0; k < kTermsPerLaunch; ++k) {
+    for (int k = 0; k < NT; ++k) {
       const double s0 = (__popc(j & mh[k]) & 1u) ? -1.0 : 1.0;
       const double s1 = (__popc((j + 1) & mh[k]) & 1u) ? -1.0 : 1.0;
       acc[k] = __builtin_fma(s1, p1, __builtin_fma(s0, p0, acc[k]));
@@ -224,8 +229,9 @@ __global__ void __launch_bounds__(256) k_term_expec_diag(const typename V2<S>::T
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int k = 0; k < kTermsPerLaunch; ++k) {
-    double s = sl[k] * acc[k];
+  for (int k = 0; k < NT; ++k) {
+    const double sl = (__popc(g & mask_of(k)) & 1u) ? -1.0 : 1.0;
+    double s = sl * acc[k];
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     if (lane == 0) red[wave][k] = s;
   }
@@ -455,19 +461,24 @@ hipError_t simple_dot(int dtype, const void* a, const void* b, int n, int mode, 
 }
 
 // out[k] = <P_k> for the nt terms t (device arrays); part needs
-// kTermBlocks * kTermsPerLaunch doubles.  h_t: a host copy of t, or NULL;
-// with it, launches of diagonal terms only take the mask kernel.
+// kTermBlocks * kTermPartRows doubles.  h_t: a host copy of t, or NULL; with
+// it, a run of up to kDiagTermsPerLaunch diagonal terms takes one launch of
+// the mask kernel, other terms go kTermsPerLaunch at a time to the general
+// kernel.  Each term's sum does not depend on the launch it is in.
 hipError_t simple_term_expec(int dtype, const void* v, int n, const PTerm* t, int nt, double* part, double* out,
                              hipStream_t st, const PTerm* h_t) {
   if (n < 1 || n > kSimpleMaxN) return hipErrorInvalidValue;
   const uint32_t N = (uint32_t)(1ull << n);
   const int blocks = (int)((N + 255) / 256) < kTermBlocks ? (int)((N + 255) / 256) : kTermBlocks;
-  for (int k0 = 0; k0 < nt; k0 += kTermsPerLaunch) {
-    const int m = nt - k0 < kTermsPerLaunch ? nt - k0 : kTermsPerLaunch;
-    bool diag = h_t != nullptr;
-    for (int k = 0; diag && k < m; ++k) diag = h_t[k0 + k].kind == 0 || h_t[k0 + k].kind == 3;
+  auto is_diag = [&](int k) { return h_t[k].kind == 0 || h_t[k].kind == 3; };
+  for (int k0 = 0, m = 0; k0 < nt; k0 += m) {
     // the diagonal kernel needs one thread per amplitude or fewer: N >= 256
-    if (diag && N >= 256) {
+    m = 0;
+    if (h_t && N >= 256)
+      while (m < kDiagTermsPerLaunch && k0 + m < nt && is_diag(k0 + m)) ++m;
+    const bool diag = m > 0;
+    if (!diag) m = nt - k0 < kTermsPerLaunch ? nt - k0 : kTermsPerLaunch;
+    if (diag) {
       int sb = 0;
       while ((1u << sb) < (uint32_t)blocks * 256u) ++sb;
       QG_DT_SWITCH(dtype, hipLaunchKernelGGL(k_term_expec_diag<S>, dim3(blocks), dim3(256), 0, st,

@@ -98,3 +98,32 @@ def test_sampled_gradients_fixture():
         E, g = seeded(fn)
         assert abs(E - d['E_' + key]) <= 1e-13, key
         assert np.max(np.abs(g - d['grad_' + key])) <= 1e-13, key
+
+
+def test_config2_fixtures_are_consistent():
+    """The 24-qubit, 40-layer fixtures (too slow to recompute here): their
+    inputs are the seed-2024 workload drawn in the reference's order, the two
+    FP64 formulations agree to the recorded 1.7e-12, and the extended-precision
+    values sit within that of both (each FP64 formulation rounds differently)."""
+    from tests.golden.make_golden import mcclean_cfg2_workload
+    d = fx.load('mcclean_n24_L40_zzchain.npz')
+    ext = fx.load('mcclean_n24_L40_zzchain_ext.npz')
+    n, L, axes, angles, w = mcclean_cfg2_workload()
+    assert np.array_equal(d['axes'], axes) and np.array_equal(d['angles'], angles)
+    assert np.array_equal(d['weights'], w)
+    assert d['grad'].shape == ext['grad'].shape == (L, n)
+    assert abs(_rel(d['grad_adjoint'], d['grad']) - float(d['formulation_rel_err'])) <= 1e-15
+    assert 1e-13 < float(d['formulation_rel_err']) <= 1e-11
+    assert _rel(d['grad'], ext['grad']) <= 1e-11 and _rel(d['grad_adjoint'], ext['grad']) <= 1e-11
+    assert abs(float(d['E']) - float(ext['E'])) <= 1e-12 and abs(float(d['E_adjoint']) - float(ext['E'])) <= 1e-12
+
+
+def test_qaoa_batch_expected_fixture_spot_check():
+    """Two of the 512 oracle results of the batched QAOA workload, recomputed
+    with the matrix-free oracle (the whole set takes minutes in C)."""
+    w = fx.load('qaoa_reg3_n20_b512.npz')
+    ref = fx.load('qaoa_reg3_n20_b512_expected.npz')
+    assert ref['E'].shape == (512,) and ref['grad'].shape == (512, 8, 2)
+    for b in (0, 511):
+        E, g = sv.qaoa_grad_run_adjoint(20, fx.maxcut_obs(20, w['edges'][b]), w['betas'][b], w['gammas'][b])
+        assert abs(E - ref['E'][b]) <= 1e-12 * max(1.0, abs(E)) and _rel(g, ref['grad'][b]) <= 1e-12

@@ -322,12 +322,6 @@ static Geo with_role_r(const Geo& g0, int dtype, int role, int r_fwd, int r_rev,
 }
 #define with_role(g0, dtype, role, tf) with_role_r(g0, dtype, role, p->r_fwd, p->r_rev, tf)
 
-static int local_of_global(const Geo& g, int b) {  // -1 if not local
-  if (b < g.c) return b;
-  if (b >= g.p && b < g.p + (g.K - g.c)) return g.c + (b - g.p);
-  return -1;
-}
-
 // Columns of the basis-index map of CNOT-ladder stacking `s` (state.py:229-241):
 // stacking 1 acts the even bonds first, stacking 0 the odd bonds first.
 static void ladder_map(int n, int stacking, uint32_t* cols) {

@@ -132,3 +132,34 @@ def test_fifo_keeps_arrival_order_on_one_slot():
     assert _order(srv, reqs) == ['first', 'big', 'small', 'medium']
     with pytest.raises(ValueError):
         GradServer(simulate=True, policy='random')
+
+
+def test_invalid_requests_fail_at_submit_through_the_c_abi():
+    """qg_submit validates before queueing (the reference's ValueErrors):
+    zz terms must be upper-triangular, angles finite, the qubit count in
+    [1, 32]; a valid request right after still runs."""
+    import ctypes
+    from qradient_amd import _native
+    srv = GradServer(slots=1, budget_bytes=1 << 30, simulate=True)
+    lib = _native.load()
+    n, L = 6, 2
+    axes = np.zeros((L, n), dtype=np.int32)
+    angles = np.zeros((L, n))
+    t = ctypes.c_int64()
+
+    def submit(n_, axes_, angles_, terms):
+        arr, k = _native.terms_array(terms)
+        return lib.qg_submit(srv._h, n_, L, _native.iptr(axes_), _native.dptr(angles_), arr, k, ctypes.byref(t))
+
+    ok_terms = [(_native.TERM_ZZ, 0, 1, 1.0)]
+    assert submit(n, axes, angles, [(_native.TERM_ZZ, 2, 1, 1.0)]) == _native.QG_ERR_INVALID   # lower triangle
+    assert submit(n, axes, angles, [(_native.TERM_ZZ, 0, n, 1.0)]) == _native.QG_ERR_INVALID   # j out of range
+    assert submit(n, axes, angles, [(_native.TERM_Z, 0, 0, float('nan'))]) == _native.QG_ERR_INVALID
+    bad = angles.copy()
+    bad[1, 3] = float('inf')
+    assert submit(n, axes, bad, ok_terms) == _native.QG_ERR_INVALID
+    assert submit(0, axes, angles, ok_terms) == _native.QG_ERR_INVALID
+    assert submit(33, np.zeros((L, 33), dtype=np.int32), np.zeros((L, 33)), ok_terms) == _native.QG_ERR_INVALID
+    assert srv.submit(_req(n, L)).result(timeout=10)[0] == float(n)
+    assert srv.counters()['queued'] == 0
+    srv.close()

@@ -761,9 +761,11 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
   // A's gates only matter for later reductions or for A's own use after the gates
   const bool a_used_after = !TWO || (flags & (F_STORE_A | F_DRED_POST | F_OBS_DIAG));
 
-  // XCD-aware tile order: workgroup blk runs on XCD blk % 8; each XCD gets a
-  // contiguous range of tiles so tiles that share cache lines share an L2.
-  const bool xcd = (nblk >= 8) && (nblk % 8 == 0) && (ntiles % nblk == 0);
+  // Tile order, chosen by the planner (F_XCD_ORDER): XCD-aware, workgroup blk
+  // runs on XCD blk % 8 and each XCD gets a contiguous range of tiles (tiles
+  // that share cache lines share an L2), or round-robin, tile blk + r * nblk
+  // (the machine's accesses stay in one moving window of memory).
+  const bool xcd = (flags & F_XCD_ORDER) && (nblk >= 8) && (nblk % 8 == 0) && (ntiles % nblk == 0);
   const uint32_t per_x = ntiles / 8, slots = nblk / 8;
   const uint32_t rounds = xcd ? ntiles / nblk : (blk < ntiles ? (ntiles - 1 - blk) / nblk + 1 : 0u);
   auto tile_of = [&](uint32_t r) -> uint32_t {

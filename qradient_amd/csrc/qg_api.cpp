@@ -54,7 +54,7 @@ struct FusedDescH {
   int32_t ndterms;
   const DTerm* xy;
   int32_t nxy;
-  const uint32_t* map;
+  uint32_t lad1, lad2;
   const double* init_tab;
   double* out;
   int32_t grad;
@@ -341,6 +341,28 @@ static void ladder_map(int n, int stacking, std::vector<uint32_t>& cols) {
   ladder_map(n, stacking, cols.data());
 }
 
+// The same map in the shift form the kernels evaluate (ladder_of):
+// map(v) = v ^ ((v & m1) >> 1) ^ ((v & m2) >> 2).  Each bond's target is the
+// bit below its control and the two groups of bonds act one after the other,
+// so column k holds bit k and at most bits k-1 and k-2; the masks are read off
+// the columns, and the form is checked column by column.
+static bool ladder_shift_form(int n, int stacking, uint32_t* m1, uint32_t* m2) {
+  std::vector<uint32_t> cols;
+  ladder_map(n, stacking, cols);
+  uint32_t a = 0, b = 0;
+  for (int k = 0; k < n; ++k) {
+    const uint32_t c = cols[k];
+    if (k >= 1 && ((c >> (k - 1)) & 1u)) a |= 1u << k;
+    if (k >= 2 && ((c >> (k - 2)) & 1u)) b |= 1u << k;
+    const uint32_t expect = (1u << k) | (((a >> k) & 1u) << (k >= 1 ? k - 1 : 0)) |
+                            (((b >> k) & 1u) << (k >= 2 ? k - 2 : 0));
+    if (c != expect) return false;
+  }
+  *m1 = a;
+  *m2 = b;
+  return true;
+}
+
 // Amplitude of |x> in the RY(pi/4) product state by popcount k = |x|
 // (mc_clean.py:55-56): cos(pi/8)^(n-k) sin(pi/8)^k, for k = 0..n.
 static void ry_init_table(int n, double* tab) {
@@ -404,7 +426,7 @@ struct qg_plan {
   bool ws_owned = false;
   size_t off_hist = 0, off_lam = 0, off_ini = 0, off_prep = 0;
   // small device tables
-  uint32_t* d_map = nullptr;    // n columns (gather/scatter)
+  uint32_t lad1 = 0, lad2 = 0;  // the ladder map in shift form (gather/scatter)
   double* d_init = nullptr;     // n + 1
   DTerm* d_terms = nullptr;     // diagonal terms (max_terms)
   std::vector<DTerm> h_terms;   // host copy of what d_terms holds for the current run
@@ -423,12 +445,12 @@ struct qg_plan {
   size_t fin_cap = 0;
   // capacities of the small tables (they only grow, so a plan re-laid out for
   // a shape no larger than one it held before allocates nothing)
-  int cap_n = 0;            // d_map (n), d_init (n + 1), h_tabs
+  int cap_n = 0;            // d_init (n + 1), h_tabs
   int cap_terms = 0;        // d_terms, d_xy
   size_t cap_part = 0;      // d_part elements (rows x gmax)
   int cap_nout = 0;         // d_scale, d_out, h_out
   size_t cap_fstate = 0;    // d_fstate bytes
-  char* h_tabs = nullptr;   // pinned staging of the map and RY(pi/4) tables
+  char* h_tabs = nullptr;   // pinned staging of the initial-state table
   double* d_scale = nullptr;
   double* d_out = nullptr;
   double* d_dot = nullptr;      // kDotBlocks + 1
@@ -559,7 +581,7 @@ static int grid_max(const qg_plan* p) {
 // tables of an n-qubit circuit); tables already large enough are kept.
 static int ensure_small(qg_plan* p, int ndesc, int nrows, int nout) {
   const int n = p->n;
-  const bool grow_n = n > p->cap_n || !p->d_map;
+  const bool grow_n = n > p->cap_n || !p->d_init;
   const bool grow_terms = std::max(64, n * n) > p->cap_terms || !p->d_terms;
   const size_t part = (size_t)std::max(1, nrows) * p->gmax;
   const size_t fin = (size_t)(3 * (nout + 1) + 3 * nrows + 8);
@@ -573,17 +595,13 @@ static int ensure_small(qg_plan* p, int ndesc, int nrows, int nout) {
     drop_graph(p);  // a captured graph holds their addresses
   }
   if (grow_n) {
-    (void)hipFree(p->d_map);
     (void)hipFree(p->d_init);
     if (p->h_tabs) (void)hipHostFree(p->h_tabs);
-    p->d_map = nullptr;
     p->d_init = nullptr;
     p->h_tabs = nullptr;
     p->cap_n = 0;
-    HIPCHK(hipMalloc(&p->d_map, sizeof(uint32_t) * std::max(n, 1)));
     HIPCHK(hipMalloc(&p->d_init, sizeof(double) * (n + 1)));
-    HIPCHK(hipHostMalloc(&p->h_tabs, sizeof(uint32_t) * std::max(n, 1) + sizeof(double) * (n + 1) + 16,
-                         hipHostMallocDefault));
+    HIPCHK(hipHostMalloc(&p->h_tabs, sizeof(double) * (n + 1), hipHostMallocDefault));
     p->cap_n = n;
   }
   if (grow_terms) {
@@ -1265,7 +1283,6 @@ static void free_small(qg_plan* p) {
   p->d_ini_states = nullptr;
   p->ini_cap = 0;
   p->cterm_cap = 0;
-  (void)hipFree(p->d_map);
   (void)hipFree(p->d_init);
   (void)hipFree(p->d_terms);
   (void)hipFree(p->d_xy);
@@ -1291,7 +1308,6 @@ static void free_small(qg_plan* p) {
   p->d_arena = nullptr;
   p->d_fstate = nullptr;
   p->arena_cap = 0;
-  p->d_map = nullptr;
   p->d_init = nullptr;
   p->d_terms = p->d_xy = nullptr;
   p->d_lut = p->d_part = p->d_scale = p->d_out = p->d_dot = nullptr;
@@ -1401,11 +1417,9 @@ static int mcclean_layout(qg_plan* p) {
   // static tables, staged in pinned memory and copied in stream order (the
   // previous run on this stream ended with a synchronisation, so the staging
   // is free and the tables are no longer read)
-  uint32_t* hm = (uint32_t*)p->h_tabs;
-  double* hi = (double*)(p->h_tabs + ((sizeof(uint32_t) * std::max(n, 1) + 15) & ~(size_t)15));
-  ladder_map(n, 1, hm);
+  if (!ladder_shift_form(n, 1, &p->lad1, &p->lad2)) return fail(QG_ERR_INVALID, "ladder map not in shift form");
+  double* hi = (double*)p->h_tabs;
   ry_init_table(n, hi);
-  HIPCHK(hipMemcpyAsync(p->d_map, hm, sizeof(uint32_t) * n, hipMemcpyHostToDevice, p->ctx->stream));
   HIPCHK(hipMemcpyAsync(p->d_init, hi, sizeof(double) * (n + 1), hipMemcpyHostToDevice, p->ctx->stream));
   HIPCHK(hipStreamSynchronize(p->ctx->stream));
   return QG_OK;
@@ -1502,7 +1516,8 @@ static int mcclean_schedule(qg_plan* p, const int32_t* axes, const double* angle
     PassDesc& d = p->h_desc[nd++];
     desc_geo(d, n, g);
     d.grid = p->gmax;  // stride of the partial rows; the launch grid is l.grid
-    d.map = p->d_map;
+    d.lad1 = p->lad1;
+    d.lad2 = p->lad2;
     d.init_tab = p->d_init;
     d.dterms = p->d_terms;
     d.ndterms = (int)obs.diag.size();
@@ -1719,7 +1734,8 @@ static int run_fused(qg_plan* p, int B, size_t bytes_per_circuit, int per_out, b
     h_desc[b].n = p->n;
     h_desc[b].L = p->L;
     h_desc[b].ent = p->ent;
-    h_desc[b].map = p->d_map;
+    h_desc[b].lad1 = p->lad1;
+    h_desc[b].lad2 = p->lad2;
     h_desc[b].init_tab = p->d_init;
     h_desc[b].out = d_out + (size_t)b * per_out;
     h_desc[b].grad = grad ? 1 : 0;
@@ -1918,11 +1934,9 @@ static int mcclean_batch_layout(qg_plan* p) {
   const int ch = n > kFusedMaxN ? p->chunk : 1;
   p->nrows = ch * (P * L + 1) * (kMaxK + 1);
   CHK(ensure_small(p, ch * 2 * P * L, p->nrows, ch * (1 + L * n)));
-  uint32_t* hm = (uint32_t*)p->h_tabs;
-  double* hi = (double*)(p->h_tabs + ((sizeof(uint32_t) * std::max(n, 1) + 15) & ~(size_t)15));
-  ladder_map(n, 1, hm);
+  if (!ladder_shift_form(n, 1, &p->lad1, &p->lad2)) return fail(QG_ERR_INVALID, "ladder map not in shift form");
+  double* hi = (double*)p->h_tabs;
   ry_init_table(n, hi);
-  HIPCHK(hipMemcpyAsync(p->d_map, hm, sizeof(uint32_t) * n, hipMemcpyHostToDevice, p->ctx->stream));
   HIPCHK(hipMemcpyAsync(p->d_init, hi, sizeof(double) * (n + 1), hipMemcpyHostToDevice, p->ctx->stream));
   HIPCHK(hipStreamSynchronize(p->ctx->stream));
   return QG_OK;
@@ -2041,7 +2055,8 @@ extern "C" int qg_mcclean_batch_grad_run(qg_plan* p, int B, const int32_t* axes,
       PassDesc& d = p->h_desc[nd++];
       desc_geo(d, n, G);
       d.grid = p->gmax;
-      d.map = p->d_map;
+      d.lad1 = p->lad1;
+      d.lad2 = p->lad2;
       d.init_tab = p->d_init;
       d.dterms = p->d_terms;
       d.ndterms = (int)obs.diag.size();
@@ -2185,7 +2200,6 @@ extern "C" int qg_mcclean_grad_run_noalloc(qg_ctx* c, int n, int L, int entangle
   const size_t o_gcs = take(sizeof(double) * 2 * np);
   const size_t o_diag = take(sizeof(DTerm) * std::max(1, ndiag));
   const size_t o_xy = take(sizeof(DTerm) * std::max(1, nxy));
-  const size_t o_map = take(sizeof(uint32_t) * n);
   const size_t o_init = take(sizeof(double) * (n + 1));
   const size_t o_out = take(sizeof(double) * (1 + np));
   if (off > kNoAllocScratchBytes)
@@ -2217,7 +2231,8 @@ extern "C" int qg_mcclean_grad_run_noalloc(qg_ctx* c, int n, int L, int entangle
     else hx[ix++] = {bi, 1, t.w};
   }
   // the ladder's gather map and the RY(pi/4) table, as a McClean plan's
-  ladder_map(n, 1, (uint32_t*)(h + o_map));
+  uint32_t lad1 = 0, lad2 = 0;
+  if (!ladder_shift_form(n, 1, &lad1, &lad2)) return fail(QG_ERR_INVALID, "ladder map not in shift form");
   ry_init_table(n, (double*)(h + o_init));
   FusedDescH* hdsc = (FusedDescH*)(h + o_desc);
   std::memset(hdsc, 0, sizeof(FusedDescH));
@@ -2231,7 +2246,8 @@ extern "C" int qg_mcclean_grad_run_noalloc(qg_ctx* c, int n, int L, int entangle
   hdsc->ndterms = ndiag;
   hdsc->xy = (const DTerm*)(dev + o_xy);
   hdsc->nxy = nxy;
-  hdsc->map = (const uint32_t*)(dev + o_map);
+  hdsc->lad1 = lad1;
+  hdsc->lad2 = lad2;
   hdsc->init_tab = (const double*)(dev + o_init);
   hdsc->out = (double*)(dev + o_out);
   hdsc->grad = 1;
@@ -2282,7 +2298,8 @@ static void* mcclean_forward_suffix(qg_plan* p, const int32_t* axes, const doubl
       PassDesc& d = p->h_desc[nd++];
       desc_geo(d, n, G);
       d.grid = p->gmax;
-      d.map = p->d_map;
+      d.lad1 = p->lad1;
+      d.lad2 = p->lad2;
       d.init_tab = p->d_init;
       d.dterms = p->d_terms;
       d.ndterms = 0;
@@ -2520,7 +2537,7 @@ static int qaoa_layout(qg_plan* p) {
     p->qaoa_terms_cap = need;
   }
   p->max_terms = mt;
-  double* hi = (double*)(p->h_tabs + ((sizeof(uint32_t) * std::max(n, 1) + 15) & ~(size_t)15));
+  double* hi = (double*)p->h_tabs;
   for (int k = 0; k <= n; ++k) hi[k] = std::pow(2.0, -0.5 * n);
   HIPCHK(hipMemcpyAsync(p->d_init, hi, sizeof(double) * (n + 1), hipMemcpyHostToDevice, p->ctx->stream));
   HIPCHK(hipStreamSynchronize(p->ctx->stream));

@@ -26,7 +26,7 @@ struct FusedDesc {
   int32_t ndterms;
   const DTerm* xy;        // x / y terms: a = bit, b = 0 (X) or 1 (Y)
   int32_t nxy;
-  const uint32_t* map;    // ladder gather map columns (CNOT), n entries
+  uint32_t lad1, lad2;    // CNOT ladder gather map in shift form (ladder_of)
   const double* init_tab; // n + 1 entries
   double* out;            // [1 + L*n] (McClean) or [1 + 2L] (QAOA)
   int32_t grad;           // 0: forward and E only
@@ -42,14 +42,14 @@ __device__ __forceinline__ void load_state_natural(Cd (&x)[1 << R], const FusedD
 
 // x(y) <- x(map(y)) over the whole state, through LDS (natural layout in and out).
 template <int R>
-__device__ __forceinline__ void permute_gather(Cd (&x)[1 << R], void* lds, const uint32_t* map, int n, uint32_t tid,
+__device__ __forceinline__ void permute_gather(Cd (&x)[1 << R], void* lds, uint32_t m1, uint32_t m2, uint32_t tid,
                                                int t) {
 #pragma unroll
   for (int a = 0; a < (1 << R); ++a) Stor<double>::lds_st(lds, swz<double>(tid | ((uint32_t)a << t)), x[a]);
   __syncthreads();
 #pragma unroll
   for (int a = 0; a < (1 << R); ++a) {
-    const uint32_t src = apply_map(map, tid | ((uint32_t)a << t), n);
+    const uint32_t src = ladder_of(tid | ((uint32_t)a << t), m1, m2);
     x[a] = Stor<double>::template lds_ld<double>(lds, swz<double>(src));
   }
   __syncthreads();
@@ -57,11 +57,11 @@ __device__ __forceinline__ void permute_gather(Cd (&x)[1 << R], void* lds, const
 
 // x(map(y)) <- x(y): the inverse of permute_gather with the same map.
 template <int R>
-__device__ __forceinline__ void permute_scatter(Cd (&x)[1 << R], void* lds, const uint32_t* map, int n, uint32_t tid,
+__device__ __forceinline__ void permute_scatter(Cd (&x)[1 << R], void* lds, uint32_t m1, uint32_t m2, uint32_t tid,
                                                 int t) {
 #pragma unroll
   for (int a = 0; a < (1 << R); ++a) {
-    const uint32_t dst = apply_map(map, tid | ((uint32_t)a << t), n);
+    const uint32_t dst = ladder_of(tid | ((uint32_t)a << t), m1, m2);
     Stor<double>::lds_st(lds, swz<double>(dst), x[a]);
   }
   __syncthreads();
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(512) void k_fused(const FusedDesc* __restrict__ des
   for (int i = 0; i < L; ++i) {
     if (d.kind == 0) {
       if (d.ent == 0) {
-        permute_gather<R>(psi, lds, d.map, n, tid, t);
+        permute_gather<R>(psi, lds, d.lad1, d.lad2, tid, t);
       } else {
 #pragma unroll
         for (int a = 0; a < (1 << R); ++a) {
@@ -238,8 +238,8 @@ __global__ __launch_bounds__(512) void k_fused(const FusedDesc* __restrict__ des
     layer_rot<R>(psi, lam, true, d, i, -1.0, lds, tid, t, red, acc, true);
     if (d.kind == 0) {
       if (d.ent == 0) {
-        permute_scatter<R>(psi, lds, d.map, n, tid, t);
-        permute_scatter<R>(lam, lds, d.map, n, tid, t);
+        permute_scatter<R>(psi, lds, d.lad1, d.lad2, tid, t);
+        permute_scatter<R>(lam, lds, d.lad1, d.lad2, tid, t);
       } else {
 #pragma unroll
         for (int a = 0; a < (1 << R); ++a) {

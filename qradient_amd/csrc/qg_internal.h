@@ -103,7 +103,9 @@ struct PassDesc {
   int32_t nx;
   int32_t xy[kMaxK];
   uint32_t zmask;
-  const uint32_t* map;   // ladder map columns (n entries), or null
+  // the ladder map (F_GATHER / F_SCATTER) in shift form:
+  // map(v) = v ^ ((v & lad1) >> 1) ^ ((v & lad2) >> 2)
+  uint32_t lad1, lad2;
   const double* init_tab;// n + 1 entries when F_INIT_TAB
   const DTerm* dterms;   // diagonal quadratic form (global bit positions a, b)
   int32_t ndterms;

@@ -286,10 +286,14 @@ __device__ __forceinline__ uint32_t gl_tile(uint32_t tt, int c, int p, int K) {
 
 __device__ __forceinline__ int gpos_of_local(int l, int c, int p) { return l < c ? l : p + (l - c); }
 
-__device__ __forceinline__ uint32_t apply_map(const uint32_t* __restrict__ m, uint32_t v, int n) {
-  uint32_t r = 0;
-  for (int k = 0; k < n; ++k) r ^= ldc(m, k) & (0u - ((v >> k) & 1u));
-  return r;
+// The CNOT-ladder index map in shift form.  Each CNOT's target is the bit just
+// below its control, and the ladder acts in two groups of disjoint bonds, so
+// the image of bit k is bit k plus at most bits k-1 and k-2: the GF(2)-linear
+// map is v ^ ((v & m1) >> 1) ^ ((v & m2) >> 2), with the masks derived on the
+// host from the ladder's columns.  Three ALU operations instead of a loop
+// over the n columns.
+__device__ __forceinline__ uint32_t ladder_of(uint32_t v, uint32_t m1, uint32_t m2) {
+  return v ^ ((v & m1) >> 1) ^ ((v & m2) >> 2);
 }
 
 __device__ __forceinline__ double cz_sign(uint32_t x, int n) {
@@ -618,15 +622,18 @@ __device__ __forceinline__ void tile_addresses(const PassDesc* __restrict__ d, b
                                                const int (&rpos)[R], const uint32_t (&rbit)[R], int n,
                                                uint32_t (&adr)[1 << R]) {
   if (gather) {
-    uint32_t mbase = apply_map(d->map, g, n) << SH;
+    // the map is linear and the register bits are disjoint from g's bits:
+    // map(g | u) = map(g) ^ map(u), with map(u) uniform
+    const uint32_t m1 = ldc(&d->lad1, 0), m2 = ldc(&d->lad2, 0);
+    uint32_t mbase = ladder_of(g, m1, m2) << SH;
     asm volatile("" : "+v"(mbase));  // opaque (see lds_transpose)
 #pragma unroll
     for (int a = 0; a < (1 << R); ++a) {
       uint32_t u = 0;
 #pragma unroll
       for (int k = 0; k < R; ++k)
-        if (a & (1 << k)) u ^= ldc(d->map, rpos[k]);
-      adr[a] = mbase ^ __builtin_amdgcn_readfirstlane(u << SH);
+        if (a & (1 << k)) u |= rbit[k];
+      adr[a] = mbase ^ __builtin_amdgcn_readfirstlane(ladder_of(u, m1, m2) << SH);
     }
   } else {
     uint32_t gb = g << SH;

@@ -924,17 +924,14 @@ static void desc_slots(PassDesc& d, int R, bool two, bool red_first) {
   // A's last gate matters only if A is used after the gates
   if (!a_used_after && last_a >= 0) d.sflag[last_a] &= ~SL_ROT_A;
   // owned thread bits of a reduce-first pass
-  d.nxy = d.nx = 0;
-  d.zmask = 0;
+  d.xmask = d.ymask = d.zmask = 0;
   if (two && red_first && (d.flags & F_REDUCE) && d.row_base >= 0) {
-    for (int pass = 0; pass < 2; ++pass)  // X bits, then Y bits
-      for (int lb = 0; lb < d.t; ++lb) {
-        if (!d.own[lb] || d.gtype[lb] == G_NONE) continue;
-        if (pass == 0 && d.gtype[lb] == G_Z) d.zmask |= 1u << lb;
-        if (d.gtype[lb] == (pass == 0 ? G_X : G_Y)) d.xy[d.nxy++] = lb;
-      }
-    for (int i = 0; i < d.nxy; ++i)
-      if (d.gtype[d.xy[i]] == G_X) d.nx = i + 1;
+    for (int lb = 0; lb < d.t; ++lb) {
+      if (!d.own[lb]) continue;
+      if (d.gtype[lb] == G_X) d.xmask |= 1u << lb;
+      if (d.gtype[lb] == G_Y) d.ymask |= 1u << lb;
+      if (d.gtype[lb] == G_Z) d.zmask |= 1u << lb;
+    }
   }
   // stages after the last one with work (only spectator bits) are skipped
   d.nstage = 1;

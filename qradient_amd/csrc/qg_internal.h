@@ -1,5 +1,6 @@
-// Internal descriptors shared by the host planner (qg_plan.cpp) and the HIP
-// kernels (qg_kernels.hip).  Plain-old-data only: the planner writes arrays of
+// Internal descriptors shared by the host planner (qg_api.cpp) and the HIP
+// kernels (qg_pass.hpp and the files that instantiate it).  Plain-old-data
+// only: the planner writes arrays of
 // these structs into device memory and every kernel reads them through a
 // pointer, so a captured hipGraph stays valid while the per-call contents
 // (angles, observable weights) change.
@@ -97,12 +98,9 @@ struct PassDesc {
   int32_t sflag[kMaxSlots];
   int32_t slb[kMaxSlots];  // local bit of the slot
   int32_t nstage;          // stages that hold a gate or a reduction (later stages are skipped)
-  // reduce-first adjoint passes: the owned thread bits (local bits below t),
-  // X / Y ones as a list of bits (the nx X bits first) and Z ones as a mask
-  int32_t nxy;
-  int32_t nx;
-  int32_t xy[kMaxK];
-  uint32_t zmask;
+  // reduce-first adjoint passes: the owned thread bits (local bits below t)
+  // by gate type, as masks of local bits
+  uint32_t xmask, ymask, zmask;
   // the ladder map (F_GATHER / F_SCATTER) in shift form:
   // map(v) = v ^ ((v & lad1) >> 1) ^ ((v & lad2) >> 2)
   uint32_t lad1, lad2;

@@ -928,7 +928,9 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
         // X / Y: the partner amplitudes of A (thread tid ^ 2^lb) from LDS,
         // raw; with FP32 tiles the next bit's reads are in flight while the
         // current bit is reduced.  The X bits and the Y bits are separate
-        // lists, so each loop has one fixed formula.
+        // masks, so each loop has one fixed formula; the bits come from the
+        // mask with scalar bit scans (no table load, whose wait would also
+        // wait for the partner reads in flight).
         typedef Cx<LT> Raw[A];
         auto rd = [&](int lb, Raw& P) __attribute__((always_inline)) {
           const uint32_t pb = base ^ swz_u<LT>(1u << lb);  // the partner thread tid ^ 2^lb
@@ -954,31 +956,47 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
           // Y: -sg * Re(...), sg = +1 where the bit is 0
           accum(lb, isx ? v : (((tid >> lb) & 1u) ? v : -v));
         };
-        auto run = [&](auto xc, int i0, int i1) __attribute__((always_inline)) {
+        // pops the lowest bit of a uniform mask
+        auto pop = [](uint32_t& m) __attribute__((always_inline)) -> int {
+          const int b = __builtin_ctz(m);
+          m &= m - 1;
+          return b;
+        };
+        auto run = [&](auto xc, uint32_t m) __attribute__((always_inline)) {
+          if (!m) return;
           if constexpr (sizeof(LT) == 4 && !WIDE) {
             Raw P0, P1;
-            if (i0 < i1) rd(ldc(d->xy, i0), P0);
-            int i = i0;
+            int b0 = pop(m);
+            rd(b0, P0);
 #pragma unroll 1
-            for (; i + 1 < i1; i += 2) {
-              rd(ldc(d->xy, i + 1), P1);
-              use(xc, ldc(d->xy, i), P0);
-              if (i + 2 < i1) rd(ldc(d->xy, i + 2), P0);
-              use(xc, ldc(d->xy, i + 1), P1);
+            while (true) {
+              if (!m) {
+                use(xc, b0, P0);
+                break;
+              }
+              const int b1 = pop(m);
+              rd(b1, P1);
+              use(xc, b0, P0);
+              if (!m) {
+                use(xc, b1, P1);
+                break;
+              }
+              b0 = pop(m);
+              rd(b0, P0);
+              use(xc, b1, P1);
             }
-            if (i < i1) use(xc, ldc(d->xy, i), P0);
           } else {
             Raw P0;
 #pragma unroll 1
-            for (int i = i0; i < i1; ++i) {
-              rd(ldc(d->xy, i), P0);
-              use(xc, ldc(d->xy, i), P0);
+            while (m) {
+              const int b = pop(m);
+              rd(b, P0);
+              use(xc, b, P0);
             }
           }
         };
-        const int nx = ldc(&d->nx, 0), nxy = ldc(&d->nxy, 0);
-        run(std::true_type{}, 0, nx);
-        run(std::false_type{}, nx, nxy);
+        run(std::true_type{}, ldc(&d->xmask, 0));
+        run(std::false_type{}, ldc(&d->ymask, 0));
         __syncthreads();  // the tile buffer is B's from here on
       }
     }

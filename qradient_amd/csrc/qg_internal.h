@@ -1,9 +1,8 @@
 // Internal descriptors shared by the host planner (qg_api.cpp) and the HIP
 // kernels (qg_pass.hpp and the files that instantiate it).  Plain-old-data
-// only: the planner writes arrays of
-// these structs into device memory and every kernel reads them through a
-// pointer, so a captured hipGraph stays valid while the per-call contents
-// (angles, observable weights) change.
+// only: the planner writes arrays of these structs into device memory and
+// every kernel reads them through a pointer, so a captured hipGraph stays
+// valid while the per-call contents (angles, observable weights) change.
 #pragma once
 #include <stdint.h>
 

@@ -522,6 +522,15 @@ __device__ __forceinline__ void static_for(F&& f) {
   static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
+// *p += v for an LDS slot that only this thread updates, as an LDS atomic
+// add without return: the wave does not wait for the slot's old value (such
+// a wait would also wait for every LDS read issued before it).  The adds of
+// one thread reach the slot in program order, so the slot holds the same
+// sum as a read-modify-write in that order.
+__device__ __forceinline__ void lds_add(double* p, double v) {
+  __builtin_amdgcn_ds_atomic_fadd_f64((__attribute__((address_space(3))) double*)p, v);
+}
+
 __device__ __forceinline__ double wave_sum(double v, int width) {
   for (int off = width >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, width);
   return v;
@@ -887,7 +896,7 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
           if (ldc(d->own, lb)) {
             const double v = red_bit<R, kk>(xb, xa, ldc(d->gtype, lb));
             if constexpr (ACCL) {
-              tacc[lb * nthr + tid] += v;
+              lds_add(&tacc[lb * nthr + tid], v);
             } else {
               const double w = wave_sum(v, width);
               if ((tid & 63) == 0) wacc[(tid >> 6) * (K + 1) + lb] += w;
@@ -914,7 +923,7 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
         __syncthreads();
         auto accum = [&](int lb, double v) __attribute__((always_inline)) {
           if constexpr (ACCL) {
-            tacc[lb * nthr + tid] += v;
+            lds_add(&tacc[lb * nthr + tid], v);
           } else {
             const double w = wave_sum(v, width);
             if ((tid & 63) == 0) wacc[(tid >> 6) * (K + 1) + lb] += w;
@@ -1025,7 +1034,7 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
               const int lb = ldc(d->slb, sl);
               const double v = red_bit<R, kk>(xb, xa, ty);
               if constexpr (ACCL) {
-                tacc[lb * nthr + tid] += v;
+                lds_add(&tacc[lb * nthr + tid], v);
               } else {
                 const double w = wave_sum(v, width);
                 if ((tid & 63) == 0) wacc[(tid >> 6) * (K + 1) + lb] += w;

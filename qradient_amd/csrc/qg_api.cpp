@@ -1565,6 +1565,8 @@ static int mcclean_schedule(qg_plan* p, const int32_t* axes, const double* angle
       void* out;
       if (p->history) {
         out = hist(i, g);
+        // a fresh history buffer per pass: its input is read once here
+        if (f & F_LOAD_A) f |= F_NT_A;
       } else if (g == P - 1 && slot[i] >= 0) {
         out = ckpt(i);                       // checkpointed layer end
       } else if (g == 0) {
@@ -2084,6 +2086,7 @@ extern "C" int qg_mcclean_batch_grad_run(qg_plan* p, int B, const int32_t* axes,
           uint32_t f = F_ROT_A | F_STORE_A;
           if (g == 0) f |= ent_fwd | (i == 0 ? F_INIT_TAB : F_LOAD_A);
           else f |= F_LOAD_A;
+          if (f & F_LOAD_A) f |= F_NT_A;  // history passes: the input is read once here
           d.a_in = (i == 0 && g == 0) ? nullptr : (g == 0 ? hist(j, i - 1, P - 1) : hist(j, i, g - 1));
           d.a_out = hist(j, i, g);
           gates(d, p->geo[g], j, i, 1.0, -1);

@@ -803,9 +803,10 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
     // non-temporal loads where it is read once per sweep, so that the lines
     // the pass writes keep the caches: the reduce-first adjoint's history
     // state, and the input of a forward pass that writes a fresh history
-    // buffer (F_NT_A; measured at 24 qubits, complex64: 57.5 -> 51.4 us per
+    // buffer small enough for the memory-side cache to keep (F_NT_A, set by
+    // the planner; measured at 24 qubits, complex64: 57.5 -> 51.4 us per
     // forward pass).  Passes that work in place or in two ping-pong buffers,
-    // which the memory-side cache holds at these sizes, read normally.
+    // and passes over larger states, read normally.
     if (load_a) {
       if (nt_a) {
         // the empty asm statements keep the two load sequences apart: without

@@ -525,14 +525,17 @@ static size_t lds_bytes_of(int dtype, int K, int threads) {  // the larger (adjo
 constexpr size_t kMaxProfRuns = 256;  // profiled runs whose launch events a plan keeps
 constexpr int kBatchWgPerCu = 16;  // workgroups per CU of a batched QAOA pass launch
 constexpr int kWgOversub = 4;      // forward pass grids: this many times the resident workgroups
-// History-mode forward passes read their input with non-temporal loads
-// (F_NT_A) when the states one launch writes fit the 256 MB memory-side
-// cache, which then keeps them for the next pass.  Measured forward passes,
-// non-temporal against ordinary loads: complex64 24 qubits 48-55 against
-// 55-59 us, 25 qubits 98.5 against 120.5 us; complex128 24 qubits 93 against
-// 105 us; but complex64 26 qubits (512 MiB) 285 against 254.5 us and
-// 27 qubits 711 against 565 us.
-constexpr size_t kNtMaxLaunchBytes = (size_t)256 << 20;
+// McClean history-mode passes read the state they read once (a forward
+// pass's input, an adjoint pass's history state) with non-temporal loads
+// (F_NT_A) while one launch covers at most 2^25 amplitudes: the lines the
+// pass writes then stay in the caches for the next pass.  Measured per pass,
+// non-temporal against ordinary loads (us): forward, complex64 24 qubits
+// 48-55 / 55-59, 25 qubits 98.5 / 120.5, complex128 24 qubits 93 / 105,
+// 25 qubits 199 / 207; adjoint, complex64 24 qubits 105.8 / 109.8,
+// 25 qubits 212.7 / 222, complex128 24 qubits 164 / 176.6, 25 qubits 329 /
+// 345.  From 26 qubits the hint costs: complex64 26 qubits forward 285 /
+// 254.5 and adjoint 494 / 469.5, 27 qubits 711 / 565 and 1117 / 1017.
+constexpr uint64_t kNtMaxLaunchAmps = (uint64_t)1 << 25;
 
 // Workgroups of this pass variant resident per CU (occupancy API, or an LDS
 // estimate where the query fails).
@@ -1574,7 +1577,7 @@ static int mcclean_schedule(qg_plan* p, const int32_t* axes, const double* angle
       if (p->history) {
         out = hist(i, g);
         // a fresh history buffer per pass: its input is read once here
-        if ((f & F_LOAD_A) && p->S <= kNtMaxLaunchBytes) f |= F_NT_A;
+        if ((f & F_LOAD_A) && ((uint64_t)1 << n) <= kNtMaxLaunchAmps) f |= F_NT_A;
       } else if (g == P - 1 && slot[i] >= 0) {
         out = ckpt(i);                       // checkpointed layer end
       } else if (g == 0) {
@@ -1645,6 +1648,7 @@ static int mcclean_schedule(qg_plan* p, const int32_t* axes, const double* angle
       }
       if (p->history) {
         d.a_in = hist(i, g);
+        if (((uint64_t)1 << n) <= kNtMaxLaunchAmps) f |= F_NT_A;  // read once here
       } else {
         const void* ain = (g == P - 1 && slot[i] >= 0) ? ckpt(i) : acur;
         void* aout = (g == 0) ? other(ain) : (void*)ain;
@@ -2095,7 +2099,7 @@ extern "C" int qg_mcclean_batch_grad_run(qg_plan* p, int B, const int32_t* axes,
           if (g == 0) f |= ent_fwd | (i == 0 ? F_INIT_TAB : F_LOAD_A);
           else f |= F_LOAD_A;
           // history passes: the input is read once here
-          if ((f & F_LOAD_A) && (size_t)nb * p->S <= kNtMaxLaunchBytes) f |= F_NT_A;
+          if ((f & F_LOAD_A) && ((uint64_t)nb << n) <= kNtMaxLaunchAmps) f |= F_NT_A;
           d.a_in = (i == 0 && g == 0) ? nullptr : (g == 0 ? hist(j, i - 1, P - 1) : hist(j, i, g - 1));
           d.a_out = hist(j, i, g);
           gates(d, p->geo[g], j, i, 1.0, -1);
@@ -2122,6 +2126,7 @@ extern "C" int qg_mcclean_batch_grad_run(qg_plan* p, int B, const int32_t* axes,
           PassDesc& d = base(G);
           uint32_t f = F_LOAD_A | F_ROT_A | F_LOAD_B | F_ROT_B | F_STORE_B | F_REDUCE;
           if (g == 0) f |= ent_rev;
+          if (((uint64_t)nb << n) <= kNtMaxLaunchAmps) f |= F_NT_A;  // the history state is read once here
           d.a_in = hist(j, i, g);
           d.b_in = lam(j, lc);
           d.b_out = lam(j, swap ? lc ^ 1 : lc);
@@ -2877,7 +2882,9 @@ static int qaoa_run(qg_plan* p, int B, const qg_term* terms, const int32_t* term
           const int gr = begin_pass(G, true, ROLE_REV_HIST);
           for (int j = 0; j < nb; ++j) {
             PassDesc& d = base_desc(cur_geo, j);
-            uint32_t f = F_LOAD_A | F_ROT_A | F_LOAD_B | F_ROT_B | F_STORE_B | F_REDUCE | phase_flag;
+            // the history state is read once here (5345-5352 circuits/s
+            // for the 512-circuit batch with ordinary loads, 5352-5378 with these)
+            uint32_t f = F_LOAD_A | F_ROT_A | F_LOAD_B | F_ROT_B | F_STORE_B | F_REDUCE | F_NT_A | phase_flag;
             d.a_in = hist(j, i, g);
             d.b_in = lam(j);
             d.b_out = lam(j);

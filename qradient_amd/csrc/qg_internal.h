@@ -46,7 +46,7 @@ enum : uint32_t {
   F_PHASE_LUT   = 1u << 16,  // D(x) is integer valued: phase from lut[D + lut_off]
   F_DRED_PRE    = 1u << 17,  // (adjoint, with F_PHASE_PRE) extra += 2*sum D(x) Im(conj(B)A) before the pre phase
   F_XCD_ORDER   = 1u << 18,  // XCD-aware tile order (else round-robin); set by the planner
-  F_NT_A        = 1u << 19,  // (forward) A is read with non-temporal loads (a history pass's input; set by the planner)
+  F_NT_A        = 1u << 19,  // A is read with non-temporal loads (read once in this sweep; set by the planner)
 };
 
 // One term of the diagonal quadratic form D(x) = sum w * z_a(x) [* z_b(x)],

@@ -774,7 +774,7 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
   constexpr bool RF = TWO && !POST;
   const bool load_a = RF || !(flags & F_INIT_TAB);
   const bool load_b = TWO && (flags & F_LOAD_B);
-  const bool nt_a = (TWO && !POST) || (!TWO && (flags & F_NT_A));
+  const bool nt_a = (flags & F_NT_A) != 0;
   // A's gates only matter for later reductions or for A's own use after the gates
   const bool a_used_after = !TWO || (flags & (F_STORE_A | F_DRED_POST | F_OBS_DIAG));
 
@@ -800,13 +800,12 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
     tile_addresses<R, BIG ? 0 : GSH>(d, gather, gl_tile(tt, c, p, K) | gl_local(tid, c, p), rpos, rbit, n, adr);
     // every path defines every register (zeros when a state is not loaded),
     // so no value is carried from one tile to the next.  A is read with
-    // non-temporal loads where it is read once per sweep, so that the lines
-    // the pass writes keep the caches: the reduce-first adjoint's history
-    // state, and the input of a forward pass that writes a fresh history
-    // buffer small enough for the memory-side cache to keep (F_NT_A, set by
-    // the planner; measured at 24 qubits, complex64: 57.5 -> 51.4 us per
-    // forward pass).  Passes that work in place or in two ping-pong buffers,
-    // and passes over larger states, read normally.
+    // non-temporal loads where the planner marks it as read once in this
+    // sweep (F_NT_A: a history-mode forward pass's input, an adjoint pass's
+    // history state), so that the lines the pass writes keep the caches
+    // (measured at 24 qubits, complex64: 57.5 -> 51.4 us per forward pass).
+    // Passes that work in place or in two ping-pong buffers, and passes over
+    // states too large for the caches to keep, read normally.
     if (load_a) {
       if (nt_a) {
         // the empty asm statements keep the two load sequences apart: without

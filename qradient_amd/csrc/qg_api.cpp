@@ -1485,9 +1485,12 @@ extern "C" int qg_mcclean_workspace_bytes(qg_ctx* ctx, int n, int L, int dtype, 
 
 // Builds the pass list of one McClean run into p->h_desc / p->launches and
 // the finalize tables; returns the descriptor count.
+// streamed: passes go to the device as they are built (stream_begin has been
+// called); otherwise the caller launches the list with run_launches.
 static int mcclean_schedule(qg_plan* p, const int32_t* axes, const double* angles, const ObsSplit& obs,
-                            bool with_ini, bool grad, int& ndesc_out) {
+                            bool with_ini, bool grad, int& ndesc_out, bool streamed) {
   const int n = p->n, L = p->L, P = (int)p->geo.size();
+  int flush_rc = QG_OK;
   const bool diag_obs = obs.xy.empty();
   p->launches.clear();
   p->obs_desc = p->adj_desc = -1;
@@ -1508,6 +1511,13 @@ static int mcclean_schedule(qg_plan* p, const int32_t* axes, const double* angle
   const uint32_t ent_flag_rev = p->ent == QG_ENT_CNOT ? F_SCATTER : F_CZ_POST;
 
   auto push = [&](const Geo& g0, bool two, int role) -> PassDesc& {
+    // every launch before this one is complete.  They go out in doubling
+    // groups (2, 2, 4, 8, ...): the device starts after the first two, and
+    // each later group holds more device work than the host takes to build
+    // the next, while a grad_run sends about log2(passes) descriptor copies
+    // (a copy between every two 50-100 us passes cost more than it hid)
+    if (streamed && flush_rc == QG_OK && p->launches.size() - p->stream_next >= std::max<size_t>(2, p->stream_next))
+      flush_rc = stream_flush(p);
     bool tf = false;
     const Geo g = with_role(g0, p->dtype, role, &tf);
     Launch l;
@@ -1629,7 +1639,7 @@ static int mcclean_schedule(qg_plan* p, const int32_t* axes, const double* angle
   p->adj_desc = nd;
   if (!grad) {
     ndesc_out = nd;
-    return QG_OK;
+    return flush_rc;
   }
   // adjoint sweep
   int lc = 0;
@@ -1665,7 +1675,7 @@ static int mcclean_schedule(qg_plan* p, const int32_t* axes, const double* angle
     }
   }
   ndesc_out = nd;
-  return QG_OK;
+  return flush_rc;
 }
 
 static int upload_finalize(qg_plan* p, const std::vector<std::vector<int>>& out_rows, const std::vector<int>& cnt,
@@ -1854,7 +1864,12 @@ static int mcclean_run(qg_plan* p, const int32_t* axes, const double* angles, co
     if (e != hipSuccess) return fail(QG_ERR_HIP, std::string("ini_state upload: ") + hipGetErrorString(e));
   }
   int nd = 0;
-  CHK(mcclean_schedule(p, axes, angles, obs, ini_state != nullptr, grad, nd));
+  // passes go to the device as they are built, so the device starts while
+  // the host builds the rest (graph replay and profiled runs launch the list
+  // at the end); only the observable's pass splits terms
+  const bool streamed = !p->use_graph && !p->profile;
+  if (streamed) CHK(stream_begin(p, std::max<size_t>(1, obs.diag.size())));
+  CHK(mcclean_schedule(p, axes, angles, obs, ini_state != nullptr, grad, nd, streamed));
   // uncompute passes of a checkpointed adjoint overwrite the final state
   if (grad && !p->history) p->last_state_ptr = nullptr;
   // rows written by each pass and the finalize table
@@ -1889,7 +1904,8 @@ static int mcclean_run(qg_plan* p, const int32_t* axes, const double* angles, co
     }
   }
   std::vector<double> scale(out_rows.size(), 1.0);
-  CHK(run_launches(p, nd));
+  if (streamed) CHK(stream_flush(p));
+  else CHK(run_launches(p, nd));
   HIPCHK(hipEventRecord(c->ev1, c->stream));
   CHK(collect_profile(p));
   CHK(upload_finalize(p, out_rows, cnt, scale));

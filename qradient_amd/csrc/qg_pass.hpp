@@ -774,7 +774,9 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
   constexpr bool RF = TWO && !POST;
   const bool load_a = RF || !(flags & F_INIT_TAB);
   const bool load_b = TWO && (flags & F_LOAD_B);
-  const bool nt_a = (flags & F_NT_A) != 0;
+  // (general-form adjoints, which rotate A and store it back, read it normally:
+  // the hint is compiled out of them, which keeps their register count)
+  const bool nt_a = !POST && (flags & F_NT_A) != 0;
   // A's gates only matter for later reductions or for A's own use after the gates
   const bool a_used_after = !TWO || (flags & (F_STORE_A | F_DRED_POST | F_OBS_DIAG));
 

@@ -22,10 +22,11 @@ _libs = {}
 
 def build():
     for out, extra in ((LIB, []), (LIB_LD, ['-DORACLE_REAL=long double'])):
-        cmd = ['gcc', '-O3', '-march=x86-64-v2', '-fPIC', '-shared', '-std=gnu11'] + extra + [SRC, '-o', out + '.tmp',
-                                                                                             '-lm']
+        # a temporary name per process: parallel test workers build at once
+        tmp = '%s.tmp%d' % (out, os.getpid())
+        cmd = ['gcc', '-O3', '-march=x86-64-v2', '-fPIC', '-shared', '-std=gnu11'] + extra + [SRC, '-o', tmp, '-lm']
         subprocess.run(cmd, check=True, capture_output=True)
-        os.replace(out + '.tmp', out)
+        os.replace(tmp, out)
     return LIB
 
 

@@ -1029,8 +1029,15 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
     }
 
     // the next tile's loads go out after everything that reads this tile's
-    // inputs, so no wait for this tile's data covers them
-    if (PF && r + 1 < rounds) fetch(tile_of(r + 1));
+    // inputs, so no wait for this tile's data covers them.  They are issued at
+    // raised wave priority, so the other wave of the SIMD, busy with its
+    // stages, does not hold them back (complex64 adjoint, 24 qubits: 101.3-101.6
+    // -> 100.3 us per pass; forward passes unchanged)
+    if (PF && r + 1 < rounds) {
+      __builtin_amdgcn_s_setprio(2);
+      fetch(tile_of(r + 1));
+      __builtin_amdgcn_s_setprio(0);
+    }
 
     // ---- stages: reductions then gates on each newly visible register bit --
 #pragma unroll 1

@@ -842,7 +842,15 @@ __device__ __forceinline__ void run_pass(const PassDesc* __restrict__ d, void* l
   if (PF && rounds > 0) fetch(tile_of(0));
   for (uint32_t r = 0; r < rounds; ++r) {
     const uint32_t tt = tile_of(r);
-    if (!PF) fetch(tt);
+    if (!PF) {
+      // complex128 phase-free passes issue the tile's loads at raised wave
+      // priority too (24 qubits: the high adjoint passes 147-148 -> 136-138
+      // us; the phase passes measured no gain and keep the default)
+      constexpr bool HI = sizeof(G) == 8 && PH == 0 && !BIG;
+      if constexpr (HI) __builtin_amdgcn_s_setprio(2);
+      fetch(tt);
+      if constexpr (HI) __builtin_amdgcn_s_setprio(0);
+    }
     uint32_t tid = tid_in;
     asm volatile("" : "+v"(tid));
     const uint32_t gthr = gl_tile(tt, c, p, K) | gl_local(tid, c, p);
